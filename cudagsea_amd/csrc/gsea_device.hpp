@@ -1,0 +1,186 @@
+// gsea_device.hpp -- device-side building blocks shared by the GSEA kernels:
+// the permutation RNG, the logarithm of the log2-ratio metrics, sort-key
+// transforms and wavefront (64-lane) primitives for CDNA4 (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsea {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Permutation RNG: the reference seeds a fast KISS generator per permutation
+// with hash rounds and shuffles the labels Fisher-Yates
+// (rngpu/rngpu.hpp:27-38,145-178,288-303; seed = perm + 42,
+// correlate_genes.cuh:50-54, configuration.cuh:29).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t seed_hash(uint32_t x) {
+    x = (x + 0x7ed55d16u) + (x << 12);
+    x = (x ^ 0xc761c23cu) ^ (x >> 19);
+    x = (x + 0x165667b1u) + (x << 5);
+    x = (x + 0xd3a2646cu) ^ (x << 9);
+    x = (x + 0xfd7046c5u) + (x << 3);
+    x = (x ^ 0xb55a4f09u) ^ (x >> 16);
+    return x;
+}
+
+struct KissState {
+    uint32_t x, y, z, w, c;
+};
+
+__device__ __forceinline__ KissState kiss_seed(uint32_t seed) {
+    KissState s;
+    s.w = seed ? seed : 0xffffffffu;
+    s.x = s.y = s.z = 0;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        s.x = seed_hash(s.w);
+        s.y = seed_hash(s.x);
+        s.z = seed_hash(s.y);
+        s.w = seed_hash(s.z);
+    }
+    s.c = 0;
+    return s;
+}
+
+__device__ __forceinline__ uint32_t kiss_next(KissState &s) {
+    s.y ^= s.y << 5;
+    s.y ^= s.y >> 7;
+    s.y ^= s.y << 22;
+    const uint32_t t = s.z + s.w + s.c;
+    s.z = s.w;
+    s.c = t >> 31;
+    s.w = t & 0x7fffffffu;
+    s.x += 1411392427u;
+    return s.x + s.y + s.w;
+}
+
+// ---------------------------------------------------------------------------
+// Logarithm of the log2-ratio metrics: log(1+f) = 2 atanh(s), s = f/(2+f),
+// x = m 2^e, m in [sqrt(1/2), sqrt(2)), ln 2 split into head and tail.  Only
+// IEEE +,-,*,/ in a fixed order (the build disables contraction), so host and
+// device produce identical bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double log_f64(double x) {
+    if (x != x) return __builtin_nan("");
+    if (x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -__builtin_inf();
+    if (x == __builtin_inf()) return x;
+    uint64_t u = __double_as_longlong(x);
+    int e = 0;
+    if ((u >> 52) == 0) {
+        x = x * 18014398509481984.0;
+        u = __double_as_longlong(x);
+        e = -54;
+    }
+    e += (int)((u >> 52) & 0x7ffu) - 1023;
+    u = (u & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m = __longlong_as_double(u);
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double R = 2.0 / 23.0;
+    R = 2.0 / 21.0 + z * R;
+    R = 2.0 / 19.0 + z * R;
+    R = 2.0 / 17.0 + z * R;
+    R = 2.0 / 15.0 + z * R;
+    R = 2.0 / 13.0 + z * R;
+    R = 2.0 / 11.0 + z * R;
+    R = 2.0 / 9.0 + z * R;
+    R = 2.0 / 7.0 + z * R;
+    R = 2.0 / 5.0 + z * R;
+    R = 2.0 / 3.0 + z * R;
+    R = z * R;
+    const double lm = f - s * (f - R);
+    const double de = (double)e;
+    return de * 6.93147180369123816490e-01 +
+           (lm + de * 1.90821492927058770002e-10);
+}
+
+__device__ __forceinline__ float log_f32(float x) {
+    if (x != x) return __builtin_nanf("");
+    return (float)log_f64((double)x);
+}
+
+__device__ __forceinline__ float log_t(float x) { return log_f32(x); }
+__device__ __forceinline__ double log_t(double x) { return log_f64(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return __builtin_sqrt(x); }
+
+// math_helpers.cuh:10-13
+template <class T>
+__device__ __forceinline__ T cuabs(T x) { return x < T(0) ? -x : x; }
+
+// ---------------------------------------------------------------------------
+// Sort keys: ascending unsigned order of the key is the stated rank order
+// (descending metric unless `asc`; -0 below +0; NaN canonicalised above +inf;
+// ties broken by ascending gene id by the stable radix passes).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t sort_key(float v, bool asc) {
+    uint32_t u = (v != v) ? 0x7fc00000u : __float_as_uint(v);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return asc ? u : ~u;
+}
+
+__device__ __forceinline__ uint64_t sort_key(double v, bool asc) {
+    uint64_t u = (v != v) ? 0x7ff8000000000000ull
+                          : (uint64_t)__double_as_longlong(v);
+    u = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    return asc ? u : ~u;
+}
+
+// |value| recovered from a key
+__device__ __forceinline__ float key_abs(uint32_t k, bool asc) {
+    uint32_t t = asc ? k : ~k;
+    uint32_t u = (t & 0x80000000u) ? (t & 0x7fffffffu) : ~t;
+    return __uint_as_float(u & 0x7fffffffu);
+}
+
+// ---------------------------------------------------------------------------
+// wavefront primitives
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ uint64_t ballot64(bool p) {
+    return (uint64_t)__ballot(p);
+}
+
+// popcount(mask & lanes below me)
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum over the 64 lanes; every lane returns lane 0's result so that a
+// floating-point total is the same value on all lanes
+template <class V>
+__device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return __shfl(v, 0, kWave);
+}
+
+// exclusive prefix sum over lanes in lane order
+template <class V>
+__device__ __forceinline__ V wave_excl_scan(V v) {
+    const int lane = lane_id();
+    V inc = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        V o = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += o;
+    }
+    const V ex = __shfl_up(inc, 1, kWave);
+    return lane == 0 ? V(0) : ex;
+}
+
+}  // namespace gsea

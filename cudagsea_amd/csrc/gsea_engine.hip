@@ -1,0 +1,771 @@
+// gsea_engine.hip -- host orchestration of the permutation-GSEA path on one
+// MI355X and the extern "C" entry points declared in include/gsea_hip.h.
+//
+// Pipeline per permutation batch (the reference's gsea_proxy loop,
+// cudaGSEA_wrapper.cu:295-372, without its per-tile host work):
+//   k_class_lists -> k_metric -> k_rank_es          (ES[P][G] in HBM)
+// then once: k_set_stats -> k_perm_extrema -> k_fwer -> FDR kernels, and
+// k_transpose when the caller wants the [G][P] matrix.
+//
+// Device memory comes from a grow-only per-device workspace, so repeated
+// calls do not allocate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gsea_hip.h"
+#include "gsea_kernels.hpp"
+
+namespace gsea {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+#define HIP_TRY(expr)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+            return fail(e_ == hipErrorOutOfMemory ? GSEA_ERR_OUT_OF_MEMORY   \
+                                                  : GSEA_ERR_HIP,            \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+static const char *kMetricNames[14] = {
+    "naive_diff_of_classes",  "naive_ratio_of_classes",
+    "naive_log2_ratio_of_classes", "stable_diff_of_classes",
+    "stable_ratio_of_classes", "stable_log2_ratio_of_classes",
+    "onepass_signal2noise",   "onepass_t_test",
+    "twopass_signal2noise",   "twopass_t_test",
+    "stable_signal2noise",    "stable_t_test",
+    "overkill_signal2noise",  "overkill_t_test"};
+
+// ---------------------------------------------------------------------------
+// per-device workspace
+// ---------------------------------------------------------------------------
+struct Buffer {
+    void *ptr = nullptr;
+    size_t cap = 0;
+};
+
+enum BufId {
+    kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
+    kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
+    kBufStats, kBufEsOut, kNumBufs
+};
+
+struct DeviceState {
+    bool init = false;
+    int lds_max = 65536;
+    int cu_count = 256;
+    Buffer bufs[kNumBufs];
+    std::vector<hipEvent_t> events;
+};
+
+static std::mutex g_mu;
+static DeviceState g_dev[64];
+
+static int current_device(int *dev) {
+    HIP_TRY(hipGetDevice(dev));
+    if (*dev < 0 || *dev >= 64) return fail(GSEA_ERR_HIP, "device id out of range");
+    return GSEA_OK;
+}
+
+static int device_state(DeviceState **out) {
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    DeviceState &d = g_dev[dev];
+    if (!d.init) {
+        int v = 0;
+        HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        d.lds_max = v;
+        HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        d.cu_count = v;
+        d.init = true;
+    }
+    *out = &d;
+    return GSEA_OK;
+}
+
+static int ensure(DeviceState *d, BufId id, size_t bytes, void **out) {
+    Buffer &b = d->bufs[id];
+    bytes = std::max<size_t>(bytes, 256);
+    if (b.cap < bytes) {
+        if (b.ptr) HIP_TRY(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.cap = 0;
+        const size_t want = bytes + bytes / 8;
+        HIP_TRY(hipMalloc(&b.ptr, want));
+        b.cap = want;
+    }
+    *out = b.ptr;
+    return GSEA_OK;
+}
+
+// events recorded around launches; durations summed per stage after sync
+struct StageClock {
+    DeviceState *d;
+    hipStream_t st;
+    std::vector<std::pair<int, std::pair<int, int>>> marks;  // stage, ev0, ev1
+    int next = 0;
+    int open_stage = -1, open_ev = -1;
+
+    int event(int *idx) {
+        if (next >= (int)d->events.size()) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            d->events.push_back(e);
+        }
+        *idx = next++;
+        HIP_TRY(hipEventRecord(d->events[*idx], st));
+        return GSEA_OK;
+    }
+    int begin(int stage) {
+        open_stage = stage;
+        return event(&open_ev);
+    }
+    int end() {
+        int e1;
+        int rc = event(&e1);
+        if (rc) return rc;
+        marks.push_back({open_stage, {open_ev, e1}});
+        return GSEA_OK;
+    }
+};
+
+static int finish_timings(StageClock &clk, gsea_timings *t, int64_t batches) {
+    if (!t) return GSEA_OK;
+    std::memset(t, 0, sizeof(*t));
+    t->perm_batches = batches;
+    if (clk.marks.empty()) return GSEA_OK;
+    double *slots[5] = {&t->ms_lists, &t->ms_metric, &t->ms_rank_es,
+                        &t->ms_stats, &t->ms_transpose};
+    for (auto &m : clk.marks) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, clk.d->events[m.second.first],
+                                    clk.d->events[m.second.second]));
+        *slots[m.first] += ms;
+    }
+    float tot = 0.f;
+    HIP_TRY(hipEventElapsedTime(&tot, clk.d->events[clk.marks.front().second.first],
+                                clk.d->events[clk.marks.back().second.second]));
+    t->ms_total = tot;
+    return GSEA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static size_t align16(size_t x) { return (x + 15) / 16 * 16; }
+
+struct Labels {
+    std::vector<uint8_t> lab;
+    int nA = 0, nB = 0;
+    int offB = 0, lstride = 0;
+};
+
+// class sizes as gsea_proxy counts them (cudaGSEA_wrapper.cu:239-251)
+static int prepare_labels(const uint8_t *labels, int64_t S, int flags,
+                          Labels *L) {
+    L->lab.assign(labels, labels + S);
+    int nA = 0, nB = 0;
+    for (auto v : L->lab) { nA += v == 0; nB += v == 1; }
+    if (flags & GSEA_FLAG_SWAP_LABELS) {
+        for (auto &v : L->lab) v = (uint8_t)((v + 1) & 1);
+        std::swap(nA, nB);
+    }
+    if (nA == 0 || nB == 0)
+        return fail(GSEA_ERR_INVALID_ARGUMENT,
+                    "both phenotype classes need at least one sample");
+    int n0 = 0;
+    for (auto v : L->lab) n0 += v == 0;
+    L->nA = nA;
+    L->nB = nB;
+    L->offB = (n0 + 7) / 8 * 8;
+    L->lstride = L->offB + ((int)S - n0 + 7) / 8 * 8;
+    return GSEA_OK;
+}
+
+template <class T, int KIND, int GT>
+static hipError_t launch_metric_gt(const MetricArgs<T> &a, bool xlds,
+                                   hipStream_t st) {
+    const int gx = (a.N + GT - 1) / GT;
+    const int gy = (a.Pb + a.ppb - 1) / a.ppb;
+    if (xlds) {
+        const size_t lds = (size_t)a.S * GT * sizeof(T);
+        hipLaunchKernelGGL((k_metric<T, KIND, GT, true>), dim3(gx, gy), dim3(256),
+                           lds, st, a);
+    } else {
+        hipLaunchKernelGGL((k_metric<T, KIND, GT, false>), dim3(gx, gy), dim3(256),
+                           0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <class T, int KIND>
+static hipError_t launch_metric_kind(const MetricArgs<T> &a, int gt, bool xlds,
+                                     hipStream_t st) {
+    if (!xlds) return launch_metric_gt<T, KIND, 64>(a, false, st);
+    if (gt == 64) return launch_metric_gt<T, KIND, 64>(a, true, st);
+    if (gt == 32) return launch_metric_gt<T, KIND, 32>(a, true, st);
+    return launch_metric_gt<T, KIND, 16>(a, true, st);
+}
+
+// X tile width: the widest of 64/32/16 genes whose [S][GT] tile fits 64 KiB
+// (two workgroups per CU); otherwise X is read from global memory.
+template <class T>
+static int metric_tile(int64_t S, bool *xlds) {
+    for (int gt : {64, 32, 16}) {
+        if ((size_t)S * gt * sizeof(T) <= 65536) { *xlds = true; return gt; }
+    }
+    *xlds = false;
+    return 64;
+}
+
+template <class T>
+static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
+                         const uint16_t *lists, int64_t Pb, int metric,
+                         T *r, DeviceState *d, hipStream_t st) {
+    MetricArgs<T> a;
+    a.X = dX; a.N = (int)N; a.S = (int)S; a.lists = lists;
+    a.lstride = L.lstride; a.nA = L.nA; a.nB = L.nB; a.offB = L.offB;
+    a.Pb = (int)Pb; a.metric = metric; a.r = r;
+    bool xlds = false;
+    const int gt = metric_tile<T>(S, &xlds);
+    const int64_t gx = (N + gt - 1) / gt;
+    // enough workgroups to fill the device several times over, each streaming
+    // as many permutations as possible over its X tile
+    const int64_t target = 8LL * d->cu_count;
+    int64_t ppb = (Pb * gx + target - 1) / target;
+    ppb = std::max<int64_t>(32, std::min<int64_t>(512, (ppb + 31) / 32 * 32));
+    a.ppb = (int)ppb;
+    hipError_t e;
+    switch (acc_kind(metric)) {
+    case kNaive: e = launch_metric_kind<T, kNaive>(a, gt, xlds, st); break;
+    case kKahan: e = launch_metric_kind<T, kKahan>(a, gt, xlds, st); break;
+    case kOnepass: e = launch_metric_kind<T, kOnepass>(a, gt, xlds, st); break;
+    case kTwopass: e = launch_metric_kind<T, kTwopass>(a, gt, xlds, st); break;
+    case kKnuth: e = launch_metric_kind<T, kKnuth>(a, gt, xlds, st); break;
+    default: e = launch_metric_kind<T, kOverkill>(a, gt, xlds, st); break;
+    }
+    HIP_TRY(e);
+    return GSEA_OK;
+}
+
+template <class T, bool ALDS, int MAXCH>
+static hipError_t launch_rank_mc(const RankArgs<T> &a, size_t lds, int grid,
+                                 hipStream_t st) {
+    auto kern = k_rank_es<T, ALDS, MAXCH>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kRankThreads), lds, st, a);
+    return hipGetLastError();
+}
+
+template <class T, bool ALDS>
+static hipError_t launch_rank_alds(const RankArgs<T> &a, int maxch, size_t lds,
+                                   int grid, hipStream_t st) {
+    switch (maxch) {
+    case 1: return launch_rank_mc<T, ALDS, 1>(a, lds, grid, st);
+    case 2: return launch_rank_mc<T, ALDS, 2>(a, lds, grid, st);
+    case 4: return launch_rank_mc<T, ALDS, 4>(a, lds, grid, st);
+    case 8: return launch_rank_mc<T, ALDS, 8>(a, lds, grid, st);
+    case 12: return launch_rank_mc<T, ALDS, 12>(a, lds, grid, st);
+    case 16: return launch_rank_mc<T, ALDS, 16>(a, lds, grid, st);
+    case 20: return launch_rank_mc<T, ALDS, 20>(a, lds, grid, st);
+    case 24: return launch_rank_mc<T, ALDS, 24>(a, lds, grid, st);
+    case 28: return launch_rank_mc<T, ALDS, 28>(a, lds, grid, st);
+    default: return launch_rank_mc<T, ALDS, 32>(a, lds, grid, st);
+    }
+}
+
+static int pick_maxch(int64_t N) {
+    const int need = (int)((N + kRankThreads - 1) / kRankThreads);
+    for (int c : {1, 2, 4, 8, 12, 16, 20, 24, 28, 32})
+        if (c >= need) return c;
+    return 32;
+}
+
+// LDS layout of k_rank_es and the number of wavefronts that score sets
+struct RankPlan {
+    bool alds = false;
+    int maxch = 1;
+    size_t lds = 0;
+    int es_waves = 16;
+    int nw = 0;
+    size_t off_ids = 0, off_cnt = 0, off_misc = 0;
+};
+
+template <class T>
+static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
+    pl->maxch = pick_maxch(N);
+    const size_t npad = (size_t)kRankThreads * pl->maxch;
+    pl->nw = (int)((npad + 31) / 32);
+    for (int alds = (sizeof(T) == 4) ? 1 : 0; alds >= 0; --alds) {
+        for (int ew = kRankWaves; ew >= 1; --ew) {
+            size_t off = 0;
+            size_t region0 = alds ? align16(npad * 4) : 0;
+            off += region0;
+            const size_t off_ids = off;
+            off += align16(npad * 2);
+            const size_t off_cnt = off;
+            off += align16(std::max<size_t>((size_t)kRankWaves * 256 * 4,
+                                            (size_t)ew * pl->nw * 4));
+            const size_t off_misc = off;
+            off += 64 * 4;
+            if (off <= (size_t)d->lds_max) {
+                pl->alds = alds != 0;
+                pl->lds = off;
+                pl->es_waves = ew;
+                pl->off_ids = off_ids;
+                pl->off_cnt = off_cnt;
+                pl->off_misc = off_misc;
+                return GSEA_OK;
+            }
+        }
+    }
+    return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
+}
+
+template <class T>
+static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
+                       const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                       int flags, T *es_pg, DeviceState *d, hipStream_t st,
+                       uint32_t *idx_out = nullptr) {
+    RankPlan pl;
+    int rc = plan_rank<T>(N, d, &pl);
+    if (rc) return rc;
+    RankArgs<T> a;
+    a.r = r; a.N = (int)N; a.Pb = (int)Pb; a.perm_base = perm_base;
+    a.G = (int)G; a.set_off = d_off; a.set_genes = d_genes; a.es = es_pg;
+    a.asc = (flags & GSEA_FLAG_ASCENDING) ? 1 : 0;
+    a.es_waves = pl.es_waves; a.nw = pl.nw;
+    a.off_ids = pl.off_ids; a.off_cnt = pl.off_cnt; a.off_misc = pl.off_misc;
+    a.a_glob = nullptr;
+    a.idx_out = idx_out;
+    if (!pl.alds && G > 0) {
+        void *p = nullptr;
+        rc = ensure(d, kBufAGlob, (size_t)Pb * N * sizeof(T), &p);
+        if (rc) return rc;
+        a.a_glob = (T *)p;
+    }
+    const int grid = (int)Pb;
+    hipError_t e = pl.alds ? launch_rank_alds<T, true>(a, pl.maxch, pl.lds, grid, st)
+                           : launch_rank_alds<T, false>(a, pl.maxch, pl.lds, grid, st);
+    HIP_TRY(e);
+    return GSEA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// statistics
+// ---------------------------------------------------------------------------
+template <class T>
+static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
+                     DeviceState *d, hipStream_t st) {
+    void *pf, *pmax, *pmin;
+    int rc = ensure(d, kBufFactor, (size_t)G * sizeof(T), &pf);
+    if (!rc) rc = ensure(d, kBufMax, (size_t)P * sizeof(T), &pmax);
+    if (!rc) rc = ensure(d, kBufMin, (size_t)P * sizeof(T), &pmin);
+    if (rc) return rc;
+    T *factor = (T *)pf, *maxv = (T *)pmax, *minv = (T *)pmin;
+    const int g = (int)G;
+    hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 255) / 256), dim3(256), 0, st,
+                       es_pg, P, g, d_stats, factor);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_perm_extrema<T>, dim3((unsigned)((P + 3) / 4)), dim3(256), 0,
+                       st, es_pg, P, g, factor, maxv, minv);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
+                       P, g, maxv, minv);
+    HIP_TRY(hipGetLastError());
+
+    // FDR: thresholds are the sorted non-NaN NES (sorted on the host: G values)
+    std::vector<T> nes(G);
+    HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<T> thr;
+    thr.reserve(G);
+    for (T v : nes)
+        if (v == v) thr.push_back(v);
+    std::sort(thr.begin(), thr.end());
+    const int n = (int)thr.size();
+    void *pthr, *phist;
+    rc = ensure(d, kBufThr, sizeof(T) * std::max(n, 1), &pthr);
+    if (!rc) rc = ensure(d, kBufHist, sizeof(unsigned long long) * (2 * (n + 1) + 2), &phist);
+    if (rc) return rc;
+    unsigned long long *hub = (unsigned long long *)phist;
+    unsigned long long *hlb = hub + (n + 1);
+    unsigned long long *signs = hlb + (n + 1);
+    if (n) HIP_TRY(hipMemcpyAsync(pthr, thr.data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(phist, 0, sizeof(unsigned long long) * (2 * (n + 1) + 2), st));
+    if (P > 1) {
+        const size_t lds = align16(sizeof(T) * n) + 2 * sizeof(uint32_t) * (n + 1);
+        const int64_t total = (P - 1) * G;
+        int blocks = (int)std::min<int64_t>((total + 255) / 256, 4LL * d->cu_count);
+        if (lds <= 65536) {
+            hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), lds, st,
+                               es_pg, P, g, factor, (const T *)pthr, n, hub, hlb, signs);
+        } else {
+            hipLaunchKernelGGL((k_fdr_hist<T, false>), dim3(blocks), dim3(256), 0, st,
+                               es_pg, P, g, factor, (const T *)pthr, n, hub, hlb, signs);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_prefix_u64, dim3(1), dim3(64), 0, st, hub, hlb, n + 1);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_fdr_final<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
+                       g, (const T *)pthr, n, hub, hlb, signs);
+    HIP_TRY(hipGetLastError());
+    return GSEA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the whole path with device-resident inputs
+// ---------------------------------------------------------------------------
+static int check_sizes(int64_t N, int64_t S, int64_t G, int64_t P, int metric) {
+    if (metric < 0 || metric > 13)
+        return fail(GSEA_ERR_UNKNOWN_METRIC, "no known metric specified");
+    if (N <= 0 || S <= 0 || P <= 0)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "empty expression data or no permutations");
+    if (G <= 0) return fail(GSEA_ERR_NO_PATHS, "no gene sets specified");
+    if (N > GSEA_MAX_GENES)
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "more genes than GSEA_MAX_GENES");
+    if (S > GSEA_MAX_SAMPLES)
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "more samples than GSEA_MAX_SAMPLES");
+    if (G > (int64_t)INT32_MAX / 2 || P > (int64_t)1 << 40)
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "too many sets or permutations");
+    return GSEA_OK;
+}
+
+// permutations per batch: the metric matrix of one batch stays below 2 GiB
+static int64_t batch_perms(int64_t N, int64_t P, size_t w) {
+    const int64_t cap = std::max<int64_t>(1, ((int64_t)2 << 30) / (N * (int64_t)w));
+    return std::min<int64_t>(P, cap);
+}
+
+template <class T>
+static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                      const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                      int64_t P, int metric, int flags, T *d_es_out, T *d_stats,
+                      hipStream_t st, gsea_timings *t) {
+    int rc = check_sizes(N, S, G, P, metric);
+    if (rc) return rc;
+    if (!dX || !labels || !d_off || !d_genes || !d_stats)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    Labels L;
+    rc = prepare_labels(labels, S, flags, &L);
+    if (rc) return rc;
+
+    const int64_t Pb = batch_perms(N, P, sizeof(T));
+    void *plab, *psig, *plists, *pr, *pes;
+    rc = ensure(d, kBufLabels, (size_t)S, &plab);
+    if (!rc) rc = ensure(d, kBufSig, (size_t)Pb * S, &psig);
+    if (!rc) rc = ensure(d, kBufLists, (size_t)Pb * L.lstride * 2, &plists);
+    if (!rc) rc = ensure(d, kBufR, (size_t)Pb * N * sizeof(T), &pr);
+    if (!rc) rc = ensure(d, kBufES, (size_t)P * G * sizeof(T), &pes);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(plab, L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
+
+    StageClock clk{d, st};
+    int64_t batches = 0;
+    for (int64_t base = 0; base < P; base += Pb, ++batches) {
+        const int64_t pb = std::min(Pb, P - base);
+        if ((rc = clk.begin(0))) return rc;
+        hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((pb + 63) / 64)), dim3(64), 0, st,
+                           (const uint8_t *)plab, (int)S, base, (int)pb, (uint8_t *)psig,
+                           L.offB, (uint16_t *)plists, L.lstride);
+        HIP_TRY(hipGetLastError());
+        if ((rc = clk.end())) return rc;
+        if ((rc = clk.begin(1))) return rc;
+        rc = launch_metric<T>(dX, N, S, L, (const uint16_t *)plists, pb, metric,
+                              (T *)pr, d, st);
+        if (rc) return rc;
+        if ((rc = clk.end())) return rc;
+        if ((rc = clk.begin(2))) return rc;
+        rc = launch_rank<T>((const T *)pr, N, pb, base, d_off, d_genes, G, flags,
+                            (T *)pes, d, st);
+        if (rc) return rc;
+        if ((rc = clk.end())) return rc;
+    }
+    if ((rc = clk.begin(3))) return rc;
+    rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st);
+    if (rc) return rc;
+    if ((rc = clk.end())) return rc;
+    if (d_es_out) {
+        if ((rc = clk.begin(4))) return rc;
+        hipLaunchKernelGGL(k_transpose<T>, dim3((unsigned)((G + 31) / 32), (unsigned)((P + 31) / 32)),
+                           dim3(32, 8), 0, st, (const T *)pes, P, (int)G, d_es_out);
+        HIP_TRY(hipGetLastError());
+        if ((rc = clk.end())) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return finish_timings(clk, t, batches);
+}
+
+// host CSR check: offsets monotone from 0, genes strictly increasing per set
+static int check_sets(const int32_t *off, const int32_t *genes, int64_t G,
+                      int64_t N) {
+    if (off[0] != 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets[0] must be 0");
+    for (int64_t s = 0; s < G; ++s) {
+        if (off[s + 1] < off[s])
+            return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
+        for (int64_t i = off[s]; i < off[s + 1]; ++i) {
+            if (genes[i] < 0 || genes[i] >= N)
+                return fail(GSEA_ERR_INVALID_ARGUMENT, "gene index out of range");
+            if (i > off[s] && genes[i] <= genes[i - 1])
+                return fail(GSEA_ERR_INVALID_ARGUMENT,
+                            "gene indices of a set must be strictly increasing");
+        }
+    }
+    return GSEA_OK;
+}
+
+template <class T>
+static int run_host(const T *exprs, int64_t N, int64_t S, const uint8_t *labels,
+                    const int32_t *off, const int32_t *genes, int64_t G, int64_t P,
+                    int metric, int flags, T *es_out, T *stats_out,
+                    gsea_timings *t) {
+    int rc = check_sizes(N, S, G, P, metric);
+    if (rc) return rc;
+    if (!exprs || !labels || !off || !genes || !stats_out)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    rc = check_sets(off, genes, G, N);
+    if (rc) return rc;
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    const int64_t total = off[G];
+    void *px, *poff, *pgen, *pst, *pes = nullptr;
+    rc = ensure(d, kBufX, (size_t)N * S * sizeof(T), &px);
+    if (!rc) rc = ensure(d, kBufOff, (size_t)(G + 1) * 4, &poff);
+    if (!rc) rc = ensure(d, kBufGenes, (size_t)std::max<int64_t>(total, 1) * 4, &pgen);
+    if (!rc) rc = ensure(d, kBufStats, (size_t)5 * G * sizeof(T), &pst);
+    if (!rc && es_out) rc = ensure(d, kBufEsOut, (size_t)G * P * sizeof(T), &pes);
+    if (rc) return rc;
+    hipStream_t st = nullptr;
+    HIP_TRY(hipMemcpy(px, exprs, (size_t)N * S * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(poff, off, (size_t)(G + 1) * 4, hipMemcpyHostToDevice));
+    if (total) HIP_TRY(hipMemcpy(pgen, genes, (size_t)total * 4, hipMemcpyHostToDevice));
+    rc = run_device<T>((const T *)px, N, S, labels, (const int32_t *)poff,
+                       (const int32_t *)pgen, G, P, metric, flags, (T *)pes,
+                       (T *)pst, st, t);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(stats_out, pst, (size_t)5 * G * sizeof(T), hipMemcpyDeviceToHost));
+    if (es_out)
+        HIP_TRY(hipMemcpy(es_out, pes, (size_t)G * P * sizeof(T), hipMemcpyDeviceToHost));
+    return GSEA_OK;
+}
+
+template <class T>
+static int metric_stage(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                        int64_t P, int64_t shift, int metric, int flags, T *d_r,
+                        hipStream_t st) {
+    int rc = check_sizes(N, S, 1, P, metric);
+    if (rc) return rc;
+    if (shift < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "negative permutation shift");
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    Labels L;
+    rc = prepare_labels(labels, S, flags, &L);
+    if (rc) return rc;
+    void *plab, *psig, *plists;
+    rc = ensure(d, kBufLabels, (size_t)S, &plab);
+    if (!rc) rc = ensure(d, kBufSig, (size_t)P * S, &psig);
+    if (!rc) rc = ensure(d, kBufLists, (size_t)P * L.lstride * 2, &plists);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(plab, L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, st,
+                       (const uint8_t *)plab, (int)S, shift, (int)P, (uint8_t *)psig,
+                       L.offB, (uint16_t *)plists, L.lstride);
+    HIP_TRY(hipGetLastError());
+    rc = launch_metric<T>(dX, N, S, L, (const uint16_t *)plists, P, metric, d_r, d, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
+template <class T>
+static int rank_es_stage(const T *d_r, int64_t N, int64_t P, const int32_t *d_off,
+                         const int32_t *d_genes, int64_t G, int flags, T *d_es,
+                         hipStream_t st) {
+    int rc = check_sizes(N, 1, G, P, 0);
+    if (rc) return rc;
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    rc = launch_rank<T>(d_r, N, P, 0, d_off, d_genes, G, flags, d_es, d, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
+template <class T>
+static int argsort_stage(const T *d_keys, int64_t N, int64_t P, int flags,
+                         uint32_t *d_index, hipStream_t st) {
+    int rc = check_sizes(N, 1, 1, P, 0);
+    if (rc) return rc;
+    if (!d_keys || !d_index) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    rc = launch_rank<T>(d_keys, N, P, 0, nullptr, nullptr, 0, flags, nullptr, d, st,
+                        d_index);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
+}  // namespace gsea
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+using namespace gsea;
+
+extern "C" {
+
+int gsea_metric_id(const char *name) {
+    if (!name) return -1;
+    for (int i = 0; i < 14; ++i)
+        if (std::strcmp(name, kMetricNames[i]) == 0) return i;
+    return -1;
+}
+
+const char *gsea_metric_name(int id) {
+    return (id >= 0 && id < 14) ? kMetricNames[id] : "";
+}
+
+const char *gsea_last_error(void) { return g_last_error.c_str(); }
+
+const char *gsea_version(void) { return "cudagsea_amd 0.1.0 (gfx950)"; }
+
+int gsea_device_count(int *count) {
+    if (!count) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    HIP_TRY(hipGetDeviceCount(count));
+    return GSEA_OK;
+}
+
+int gsea_get_device(int *device) {
+    if (!device) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    HIP_TRY(hipGetDevice(device));
+    return GSEA_OK;
+}
+
+int gsea_set_device(int device) {
+    HIP_TRY(hipSetDevice(device));
+    return GSEA_OK;
+}
+
+int gsea_device_name(int device, char *name, size_t name_len, size_t *total_bytes) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s", prop.name);
+    }
+    if (total_bytes) *total_bytes = prop.totalGlobalMem;
+    return GSEA_OK;
+}
+
+int gsea_run_f32(const float *exprs, int64_t N, int64_t S, const uint8_t *labels,
+                 const int32_t *off, const int32_t *genes, int64_t G, int64_t P,
+                 int metric, int flags, float *es_out, float *stats_out,
+                 gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_host<float>(exprs, N, S, labels, off, genes, G, P, metric, flags,
+                           es_out, stats_out, t);
+}
+
+int gsea_run_f64(const double *exprs, int64_t N, int64_t S, const uint8_t *labels,
+                 const int32_t *off, const int32_t *genes, int64_t G, int64_t P,
+                 int metric, int flags, double *es_out, double *stats_out,
+                 gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_host<double>(exprs, N, S, labels, off, genes, G, P, metric, flags,
+                            es_out, stats_out, t);
+}
+
+int gsea_run_device_f32(const float *dX, int64_t N, int64_t S, const uint8_t *labels,
+                        const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                        int64_t P, int metric, int flags, float *d_es_out,
+                        float *d_stats, void *stream, gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_device<float>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
+                             d_es_out, d_stats, (hipStream_t)stream, t);
+}
+
+int gsea_run_device_f64(const double *dX, int64_t N, int64_t S, const uint8_t *labels,
+                        const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                        int64_t P, int metric, int flags, double *d_es_out,
+                        double *d_stats, void *stream, gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_device<double>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
+                              d_es_out, d_stats, (hipStream_t)stream, t);
+}
+
+int gsea_metric_f32(const float *dX, int64_t N, int64_t S, const uint8_t *labels,
+                    int64_t P, int64_t shift, int metric, int flags, float *d_r,
+                    void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return metric_stage<float>(dX, N, S, labels, P, shift, metric, flags, d_r,
+                               (hipStream_t)stream);
+}
+
+int gsea_metric_f64(const double *dX, int64_t N, int64_t S, const uint8_t *labels,
+                    int64_t P, int64_t shift, int metric, int flags, double *d_r,
+                    void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return metric_stage<double>(dX, N, S, labels, P, shift, metric, flags, d_r,
+                                (hipStream_t)stream);
+}
+
+int gsea_rank_es_f32(const float *d_r, int64_t N, int64_t P, const int32_t *d_off,
+                     const int32_t *d_genes, int64_t G, int flags, float *d_es,
+                     void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return rank_es_stage<float>(d_r, N, P, d_off, d_genes, G, flags, d_es,
+                                (hipStream_t)stream);
+}
+
+int gsea_rank_es_f64(const double *d_r, int64_t N, int64_t P, const int32_t *d_off,
+                     const int32_t *d_genes, int64_t G, int flags, double *d_es,
+                     void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return rank_es_stage<double>(d_r, N, P, d_off, d_genes, G, flags, d_es,
+                                 (hipStream_t)stream);
+}
+
+int gsea_argsort_f32(const float *d_keys, int64_t N, int64_t P, int flags,
+                     uint32_t *d_index, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return argsort_stage<float>(d_keys, N, P, flags, d_index, (hipStream_t)stream);
+}
+
+int gsea_argsort_f64(const double *d_keys, int64_t N, int64_t P, int flags,
+                     uint32_t *d_index, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return argsort_stage<double>(d_keys, N, P, flags, d_index, (hipStream_t)stream);
+}
+
+}  // extern "C"

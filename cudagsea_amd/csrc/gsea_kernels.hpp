@@ -1,0 +1,845 @@
+// gsea_kernels.hpp -- CDNA4 (gfx950) HIP kernels of the permutation-GSEA hot
+// path:
+//   k_class_lists  permutation p -> ordered sample lists of both classes
+//   k_metric       14 local ranking metrics, many permutations per X tile
+//   k_rank_es      per permutation: LDS radix argsort of all genes, then the
+//                  running-sum enrichment score of every gene set
+//   k_set_stats / k_perm_extrema / k_fwer / k_fdr_*   final statistics
+//   k_transpose    [P][G] -> [G][P] for the reference's ES dump layout
+#pragma once
+
+#include "gsea_device.hpp"
+
+namespace gsea {
+
+// ===========================================================================
+// k_class_lists: one thread per permutation.  Labels of global permutation
+// gp are the observed labels (gp == 0) or their Fisher-Yates shuffle seeded
+// with gp + 42 (correlate_genes.cuh:44-55, rngpu.hpp:288-303).  The kernel
+// writes, per permutation, the samples of class A (label 0, functors.cuh:110)
+// in increasing sample order at [0, nA) and all other samples in increasing
+// order at [offB, offB + S - nA): the metric kernels then visit each class in
+// exactly the reference's summation order.
+// ===========================================================================
+__global__ void k_class_lists(const uint8_t *__restrict__ labels, int S,
+                              int64_t perm_base, int Pb,
+                              uint8_t *__restrict__ sig, int offB,
+                              uint16_t *__restrict__ lists, int lstride) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Pb) return;
+    uint8_t *s = sig + (size_t)p * S;
+    for (int i = 0; i < S; ++i) s[i] = labels[i];
+    const uint64_t gp = (uint64_t)(perm_base + p);
+    if (gp) {
+        KissState st = kiss_seed((uint32_t)(gp + 42u));
+        for (int i = S - 1; i >= 1; --i) {
+            // (double)rng * (i+1) / 2^32 truncated == (rng * (i+1)) >> 32
+            const uint32_t r = (uint32_t)(((uint64_t)kiss_next(st) *
+                                           (uint64_t)(i + 1)) >> 32);
+            const uint8_t t = s[i];
+            s[i] = s[r];
+            s[r] = t;
+        }
+    }
+    uint16_t *L = lists + (size_t)p * lstride;
+    int a = 0, b = offB;
+    for (int i = 0; i < S; ++i) {
+        if (s[i] == 0) L[a++] = (uint16_t)i;
+        else L[b++] = (uint16_t)i;
+    }
+}
+
+// ===========================================================================
+// k_metric: local ranking metrics (functors.cuh:10-433).
+//
+// Block = 4 wavefronts.  The block stages X[0:S][g0:g0+GT] (sample-major
+// input) in LDS once and streams `ppb` permutations over it.  A wavefront is
+// split into 64/GT lane groups; lane `col` of a group owns gene g0+col, and
+// each lane carries PL permutations at once (independent accumulation
+// chains).  Every class sum walks that class's samples in increasing sample
+// order, one sample at a time, with the reference's exact arithmetic, so the
+// metric is bit-identical to the reference CPU path.
+// ===========================================================================
+enum AccKind { kNaive = 0, kKahan = 1, kOnepass = 2, kTwopass = 3,
+               kKnuth = 4, kOverkill = 5 };
+
+__host__ __device__ constexpr int acc_kind(int metric) {
+    return metric <= 2 ? kNaive
+         : metric <= 5 ? kKahan
+         : 2 + (metric - 6) / 2;
+}
+
+template <class T>
+struct MetricArgs {
+    const T *X;            // [S][N] sample-major
+    int N, S;
+    const uint16_t *lists; // [Pb][lstride]
+    int lstride, nA, nB, offB;
+    int Pb, ppb;
+    int metric;            // 0..13, selects the combiner
+    T *r;                  // [Pb][N]
+};
+
+// visit list entries [off, off+cnt) of PL permutations in order; f(k, s, i)
+template <int PL, class F>
+__device__ __forceinline__ void walk_lists(const uint16_t *const *L, int off,
+                                           int cnt, F &&f) {
+    int i = 0;
+    for (; i + 8 <= cnt; i += 8) {
+        uint4 w[PL];
+#pragma unroll
+        for (int k = 0; k < PL; ++k)
+            w[k] = *reinterpret_cast<const uint4 *>(L[k] + off + i);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+#pragma unroll
+            for (int k = 0; k < PL; ++k) {
+                const uint32_t word = j < 2 ? w[k].x : j < 4 ? w[k].y
+                                    : j < 6 ? w[k].z : w[k].w;
+                const int s = (j & 1) ? (int)(word >> 16) : (int)(word & 0xffffu);
+                f(k, s, i + j);
+            }
+        }
+    }
+    for (; i < cnt; ++i) {
+#pragma unroll
+        for (int k = 0; k < PL; ++k) f(k, (int)L[k][off + i], i);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T combine_means(int metric, T mA, T mB) {
+    const int c = metric % 3;
+    if (c == 0) return mA - mB;                       // functors.cuh:10-15
+    if (c == 1) return mA / mB;                       // functors.cuh:17-22
+    return (T)((double)log_t(mA / mB) * 1.44269504089);  // functors.cuh:24-29
+}
+
+template <class T>
+__device__ __forceinline__ T combine_stats(int metric, T mA, T vA, T nA,
+                                           T mB, T vB, T nB) {
+    if ((metric & 1) == 0) {                          // functors.cuh:44-51
+        T sA = sqrt_t(vA), sB = sqrt_t(vB);
+        const T minA = (T)(0.2 * (double)cuabs(mA));
+        const T minB = (T)(0.2 * (double)cuabs(mB));
+        sA = minA < sA ? sA : minA;
+        sB = minB < sB ? sB : minB;
+        return (mA - mB) / (sA + sB);
+    }
+    const T minA = (T)(0.04 * (double)cuabs(mA));     // functors.cuh:70-77
+    const T minB = (T)(0.04 * (double)cuabs(mB));
+    vA = minA < vA ? vA : minA;
+    vB = minB < vB ? vB : minB;
+    return (mA - mB) / sqrt_t(vA / nA + vB / nB);
+}
+
+template <class T, int KIND, int GT, bool XLDS>
+__global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
+    constexpr int PL = 4;
+    constexpr int PPW = kWave / GT;   // lane groups per wavefront
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *tile = reinterpret_cast<T *>(smem_raw);
+
+    const int g0 = blockIdx.x * GT;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int grp = lane / GT, col = lane % GT;
+    const int g = g0 + col;
+    const int gc = g < a.N ? g : a.N - 1;
+
+    const T *xs;
+    int xstride;
+    if (XLDS) {
+        for (int idx = threadIdx.x; idx < a.S * GT; idx += blockDim.x) {
+            const int s = idx / GT, j = idx % GT;
+            const int gg = g0 + j;
+            tile[idx] = gg < a.N ? a.X[(size_t)s * a.N + gg] : T(0);
+        }
+        __syncthreads();
+        xs = tile + col;
+        xstride = GT;
+    } else {
+        xs = a.X + gc;
+        xstride = a.N;
+    }
+
+    // naive sums see 0*x of the other class (functors.cuh:115-116), which
+    // turns into NaN for a non-finite x: track whether the gene has one
+    bool nonfinite = false;
+    if (KIND == kNaive) {
+        for (int s = 0; s < a.S; ++s) {
+            const T v = xs[(size_t)s * xstride];
+            nonfinite |= !(v - v == T(0));
+        }
+    }
+
+    const T nA = (T)a.nA, nB = (T)a.nB;
+    const int nRest = a.S - a.nA;
+    const int p_begin = blockIdx.y * a.ppb;
+    const int p_end = min(a.Pb, p_begin + a.ppb);
+    const int slot = wave * PPW + grp;
+    const int step = 4 * PPW * PL;
+    auto X_at = [&](int s) -> T { return xs[(size_t)s * xstride]; };
+
+    for (int base = p_begin + slot * PL; base < p_end; base += step) {
+        const uint16_t *L[PL];
+        int pk[PL];
+#pragma unroll
+        for (int k = 0; k < PL; ++k) {
+            pk[k] = base + k;
+            const int pe = pk[k] < p_end ? pk[k] : p_end - 1;
+            L[k] = a.lists + (size_t)pe * a.lstride;
+        }
+        T out[PL];
+        if (KIND == kNaive || KIND == kOnepass || KIND == kTwopass) {
+            T sA[PL], sB[PL], qA[PL], qB[PL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k) sA[k] = sB[k] = qA[k] = qB[k] = T(0);
+            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                const T v = X_at(s);
+                sA[k] = sA[k] + v;
+                if (KIND == kOnepass) qA[k] = qA[k] + v * v;
+            });
+            walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
+                const T v = X_at(s);
+                sB[k] = sB[k] + v;
+                if (KIND == kOnepass) qB[k] = qB[k] + v * v;
+            });
+            if (KIND == kNaive && nonfinite) {
+                bool bA[PL], bB[PL];
+#pragma unroll
+                for (int k = 0; k < PL; ++k) bA[k] = bB[k] = false;
+                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                    const T v = X_at(s);
+                    bA[k] |= !(v - v == T(0));
+                });
+                walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
+                    const T v = X_at(s);
+                    bB[k] |= !(v - v == T(0));
+                });
+#pragma unroll
+                for (int k = 0; k < PL; ++k) {
+                    if (bB[k]) sA[k] = (T)__builtin_nan("");
+                    if (bA[k]) sB[k] = (T)__builtin_nan("");
+                }
+            }
+            T mA[PL], mB[PL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k) { mA[k] = sA[k] / nA; mB[k] = sB[k] / nB; }
+            if (KIND == kNaive) {
+#pragma unroll
+                for (int k = 0; k < PL; ++k)
+                    out[k] = combine_means<T>(a.metric, mA[k], mB[k]);
+            } else if (KIND == kOnepass) {
+                // functors.cuh:212-218 (biased)
+#pragma unroll
+                for (int k = 0; k < PL; ++k) {
+                    const T vA = qA[k] / nA - mA[k] * mA[k];
+                    const T vB = qB[k] / nB - mB[k] * mB[k];
+                    out[k] = combine_stats<T>(a.metric, mA[k], vA, nA, mB[k], vB, nB);
+                }
+            } else {
+                // functors.cuh:270-290
+                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                    const T d = X_at(s) - mA[k];
+                    qA[k] = qA[k] + d * d;
+                });
+                walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
+                    const T d = X_at(s) - mB[k];
+                    qB[k] = qB[k] + d * d;
+                });
+#pragma unroll
+                for (int k = 0; k < PL; ++k)
+                    out[k] = combine_stats<T>(a.metric, mA[k], qA[k] / nA, nA,
+                                              mB[k], qB[k] / nB, nB);
+            }
+        } else if (KIND == kKahan || KIND == kOverkill) {
+            // Kahan mean, functors.cuh:144-172 / 316-343
+            T sA[PL], sB[PL], cA[PL], cB[PL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k) sA[k] = sB[k] = cA[k] = cB[k] = T(0);
+            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                const T alpha = X_at(s) - cA[k];
+                const T beta = sA[k] + alpha;
+                cA[k] = (beta - sA[k]) - alpha;
+                sA[k] = beta;
+            });
+            walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
+                const T alpha = X_at(s) - cB[k];
+                const T beta = sB[k] + alpha;
+                cB[k] = (beta - sB[k]) - alpha;
+                sB[k] = beta;
+            });
+            T mA[PL], mB[PL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k) { mA[k] = sA[k] / nA; mB[k] = sB[k] / nB; }
+            if (KIND == kKahan) {
+#pragma unroll
+                for (int k = 0; k < PL; ++k)
+                    out[k] = combine_means<T>(a.metric, mA[k], mB[k]);
+            } else {
+                // compensated second pass, functors.cuh:346-370
+                T uA[PL], uB[PL], vA[PL], vB[PL];
+#pragma unroll
+                for (int k = 0; k < PL; ++k) uA[k] = uB[k] = vA[k] = vB[k] = T(0);
+                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                    const T d = X_at(s) - mA[k];
+                    uA[k] = uA[k] + d;
+                    vA[k] = vA[k] + d * d;
+                });
+                walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
+                    const T d = X_at(s) - mB[k];
+                    uB[k] = uB[k] + d;
+                    vB[k] = vB[k] + d * d;
+                });
+#pragma unroll
+                for (int k = 0; k < PL; ++k) {
+                    T va = vA[k] - uA[k] * uA[k] / nA;
+                    T vb = vB[k] - uB[k] * uB[k] / nB;
+                    out[k] = combine_stats<T>(a.metric, mA[k], va / nA, nA,
+                                              mB[k], vb / nB, nB);
+                }
+            }
+        } else {
+            // Welford/Knuth, functors.cuh:396-428: the i-th sample of a class
+            // divides by its running count i+1
+            T mA[PL], mB[PL], vA[PL], vB[PL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k) mA[k] = mB[k] = vA[k] = vB[k] = T(0);
+            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int i) {
+                const T v = X_at(s);
+                const T delta = v - mA[k];
+                mA[k] = mA[k] + delta / (T)(i + 1);
+                vA[k] = vA[k] + delta * (v - mA[k]);
+            });
+            walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int i) {
+                const T v = X_at(s);
+                const T delta = v - mB[k];
+                mB[k] = mB[k] + delta / (T)(i + 1);
+                vB[k] = vB[k] + delta * (v - mB[k]);
+            });
+#pragma unroll
+            for (int k = 0; k < PL; ++k)
+                out[k] = combine_stats<T>(a.metric, mA[k], vA[k] / nA, nA,
+                                          mB[k], vB[k] / nB, nB);
+        }
+        if (g < a.N) {
+#pragma unroll
+            for (int k = 0; k < PL; ++k)
+                if (pk[k] < p_end) a.r[(size_t)pk[k] * a.N + g] = out[k];
+        }
+    }
+}
+
+// ===========================================================================
+// k_rank_es: one workgroup (16 wavefronts) per permutation.
+//
+// Rank: stable LSD radix argsort in LDS, 8-bit digits.  Keys stay in place
+// (indexed by gene); only 16-bit gene ids move.  Each wavefront ranks its
+// contiguous segment chunk by chunk: lanes with equal digits are found with
+// eight ballots, the lowest such lane bumps the wavefront's LDS counter for
+// that digit, so ranks follow element order and each pass is stable.  A
+// digit-major scan over (digit, wavefront) gives the scatter offsets.
+//
+// ES: ranks are inverted (pos[gene]) and |metric| is laid out by rank.  Each
+// wavefront takes gene sets from an LDS work counter.  For a set of k genes
+// it ORs the k rank bits into a private N-bit bitmap, scans the bitmap in
+// lane-contiguous word ranges, and evaluates the running sum only at the
+// hits: just before hit j at rank q the sum is S_{j-1} + (q-j)*decay, just
+// after it S_j + (q-j)*decay, with S_j the prefix sum of |r|/N_R over hits in
+// rank order.  The maximum deviation of the reference's walk
+// (enrichment_scores.cuh:310-330) is attained at one of these points; the
+// first strictly largest |value| in rank order wins, as in the walk.  Sums
+// are carried in double and each candidate is rounded to T before it is
+// compared, as the reference compares T values.
+// ===========================================================================
+constexpr int kRankWaves = 16;
+constexpr int kRankThreads = kRankWaves * kWave;
+// a wavefront ranks at most MAXCH chunks of 64 keys per pass, so one
+// workgroup sorts up to 16 * MAXCH * 64 keys (MAXCH <= 32: N <= 32768)
+constexpr int kMaxRankN = kRankWaves * 32 * kWave;
+
+template <class T>
+struct RankArgs {
+    const T *r;              // [Pb][N]
+    int N, Pb;
+    int64_t perm_base;
+    int G;
+    const int32_t *set_off;  // [G+1]
+    const int32_t *set_genes;
+    T *es;                   // [P][G]
+    T *a_glob;               // [Pb][N] |metric| by rank when it is not in LDS
+    uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
+    int asc;
+    int es_waves, nw;        // wavefronts that score sets, bitmap words
+    size_t off_ids, off_cnt, off_misc;
+};
+
+template <class T>
+struct KeyOf;
+template <>
+struct KeyOf<float> { typedef uint32_t type; };
+template <>
+struct KeyOf<double> { typedef uint64_t type; };
+
+template <class T, bool ALDS, int MAXCH>
+__global__ __launch_bounds__(kRankThreads) void k_rank_es(RankArgs<T> a) {
+    typedef typename KeyOf<T>::type KeyT;
+    constexpr int NPASS = (int)sizeof(KeyT);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // LDS: [keys | a_rank] (ALDS only) | ids -> pos | counters -> bitmaps | misc
+    uint32_t *keys = reinterpret_cast<uint32_t *>(smem_raw);
+    float *a_lds = reinterpret_cast<float *>(smem_raw);
+    uint16_t *ids = reinterpret_cast<uint16_t *>(smem_raw + a.off_ids);
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(smem_raw + a.off_cnt);
+    int *misc = reinterpret_cast<int *>(smem_raw + a.off_misc);
+
+    const int N = a.N;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const bool asc = a.asc != 0;
+
+    // keys are padded to Npad = 16 wavefronts x CH chunks x 64 lanes with a
+    // sentinel above every real key (NaN is canonicalised, so no metric maps
+    // to all ones); padding sorts after the N genes and is never read back
+    constexpr int CH = MAXCH;           // the host picks MAXCH >= N / 1024
+    const int seg = CH * kWave;
+    const int Npad = kRankWaves * seg;
+    const KeyT kSentinel = ~(KeyT)0;
+
+    for (int p = blockIdx.x; p < a.Pb; p += gridDim.x) {
+        const T *rp = a.r + (size_t)p * N;
+        for (int g = tid; g < Npad; g += kRankThreads) {
+            if (ALDS) keys[g] = g < N ? (uint32_t)sort_key(rp[g], asc) : 0xffffffffu;
+            ids[g] = (uint16_t)g;
+        }
+        if (tid == 0) misc[0] = 0;
+        __syncthreads();
+
+        auto key_of = [&](int g) -> KeyT {
+            if (ALDS) return (KeyT)keys[g];
+            return g < N ? (KeyT)sort_key(rp[g], asc) : kSentinel;
+        };
+
+        const int start = w * seg;
+        uint32_t *wcnt = cnt + w * 256;
+
+        for (int pass = 0; pass < NPASS; ++pass) {
+            const int shift = pass * 8;
+            for (int i = lane; i < 256; i += kWave) wcnt[i] = 0;
+            // per chunk: gene id (bits 0-15) | digit (bits 16-23); local rank
+            uint32_t pk[MAXCH], lr[MAXCH];
+#pragma unroll
+            for (int c = 0; c < MAXCH; ++c) {
+                if (c < CH) {
+                    const uint32_t id = ids[start + c * kWave + lane];
+                    const uint32_t d = (uint32_t)((key_of((int)id) >> shift) & 255u);
+                    pk[c] = id | (d << 16);
+                    // keep one packed register per chunk (no hoisted parts)
+                    asm volatile("" : "+v"(pk[c]));
+                }
+            }
+            wave_sync_lds();
+#pragma unroll
+            for (int c = 0; c < MAXCH; ++c) {
+                if (c < CH) {
+                    const uint32_t d = pk[c] >> 16;
+                    uint64_t peers = ~0ull;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const bool bit = (d >> b) & 1u;
+                        const uint64_t bal = ballot64(bit);
+                        peers &= bit ? bal : ~bal;
+                    }
+                    const int before = mbcnt64(peers);
+                    const int tot = __popcll(peers);
+                    const uint32_t base = wcnt[d];
+                    lr[c] = base + (uint32_t)before;
+                    asm volatile("" : "+v"(lr[c]));
+                    wave_sync_lds();
+                    if (before == 0) wcnt[d] = base + (uint32_t)tot;
+                    wave_sync_lds();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            __syncthreads();
+            // exclusive scan over (digit, wavefront), digit-major
+            uint32_t tot_d = 0;
+            if (tid < 256) {
+                uint32_t run = 0;
+                for (int w2 = 0; w2 < kRankWaves; ++w2) {
+                    const uint32_t v = cnt[w2 * 256 + tid];
+                    cnt[w2 * 256 + tid] = run;
+                    run += v;
+                }
+                tot_d = run;
+                uint32_t inc = tot_d;
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const uint32_t o = __shfl_up(inc, off, kWave);
+                    if (lane >= off) inc += o;
+                }
+                if (lane == kWave - 1) misc[4 + w] = (int)inc;
+                tot_d = inc - tot_d;   // exclusive within the wavefront
+            }
+            __syncthreads();
+            if (tid < 256) {
+                uint32_t base = tot_d;
+                for (int w2 = 0; w2 < w; ++w2) base += (uint32_t)misc[4 + w2];
+                for (int w2 = 0; w2 < kRankWaves; ++w2) cnt[w2 * 256 + tid] += base;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < MAXCH; ++c) {
+                if (c < CH)
+                    ids[wcnt[pk[c] >> 16] + lr[c]] = (uint16_t)(pk[c] & 0xffffu);
+            }
+            __syncthreads();
+        }
+
+        if (a.idx_out) {
+            for (int i = tid; i < N; i += kRankThreads)
+                a.idx_out[(size_t)p * N + i] = ids[i];
+        }
+        if (a.G == 0) {
+            __syncthreads();
+            continue;
+        }
+
+        // invert: pos[gene] = rank; |metric| laid out by rank (read from
+        // global memory: the LDS keys are overwritten by a_lds)
+        {
+            uint32_t gi[MAXCH];
+#pragma unroll
+            for (int j = 0; j < MAXCH; ++j) {
+                const int i = tid + j * kRankThreads;
+                const uint32_t g = ids[i];
+                gi[j] = g;
+                const T av = g < (uint32_t)N ? (T)__builtin_fabs((double)rp[g]) : T(0);
+                if (ALDS) a_lds[i] = (float)av;
+                else if (i < N) a.a_glob[(size_t)p * N + i] = av;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < MAXCH; ++j)
+                ids[gi[j]] = (uint16_t)(tid + j * kRankThreads);
+            __syncthreads();
+        }
+        const uint16_t *pos = ids;
+        const T *a_rank = ALDS ? nullptr : a.a_glob + (size_t)p * N;
+        auto A = [&](int q) -> T {
+            if (ALDS) return (T)a_lds[q];
+            return a_rank[q];
+        };
+
+        if (w < a.es_waves) {
+            uint32_t *bm = cnt + (size_t)w * a.nw;
+            for (int i = lane; i < a.nw; i += kWave) bm[i] = 0u;
+            wave_sync_lds();
+            const int W = (a.nw + kWave - 1) / kWave;
+            const int w0 = lane * W;
+            const int w1 = min(w0 + W, a.nw);
+            const int64_t gp = a.perm_base + p;
+            for (;;) {
+                int s = 0;
+                if (lane == 0) s = atomicAdd(&misc[0], 1);
+                s = __shfl(s, 0, kWave);
+                if (s >= a.G) break;
+                const int o0 = a.set_off[s];
+                const int k = a.set_off[s + 1] - o0;
+                double nr = 0.0;
+                for (int j = lane; j < k; j += kWave) {
+                    const int q = pos[a.set_genes[o0 + j]];
+                    atomicOr(&bm[q >> 5], 1u << (q & 31));
+                    nr += (double)A(q);
+                }
+                nr = wave_sum(nr);
+                const T NRt = (T)nr;
+                const T decay = (T)(-1.0 / (double)(N - k));
+                wave_sync_lds();
+                int hits = 0;
+                double cs = 0.0;
+                for (int wi = w0; wi < w1; ++wi) {
+                    uint32_t bits = bm[wi];
+                    hits += __popc(bits);
+                    while (bits) {
+                        const int q = wi * 32 + __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        const T c = A(q) / NRt;
+                        cs += (double)c;
+                    }
+                }
+                const int j0 = wave_excl_scan(hits);
+                double Sd = wave_excl_scan(cs);
+                int jj = j0;
+                T best = T(0);
+                for (int wi = w0; wi < w1; ++wi) {
+                    uint32_t bits = bm[wi];
+                    bm[wi] = 0u;
+                    while (bits) {
+                        const int q = wi * 32 + __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        const int m = q - jj;
+                        const double md = m ? (double)m * (double)decay : 0.0;
+                        const T vb = (T)(Sd + md);
+                        if (cuabs(vb) > cuabs(best)) best = vb;
+                        const T c = A(q) / NRt;
+                        Sd += (double)c;
+                        const T va = (T)(Sd + md);
+                        if (cuabs(va) > cuabs(best)) best = va;
+                        ++jj;
+                    }
+                }
+                // leftmost strict maximum of |best| across lanes (rank order)
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const T o = __shfl_down(best, off, kWave);
+                    if (lane + off < kWave && cuabs(o) > cuabs(best)) best = o;
+                }
+                if (k == 0) best = (T)((double)N * (double)decay);
+                if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
+                wave_sync_lds();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
+// statistics, statistics.cuh:46-117 (+ FDR defined by this project)
+// es is [P][G] (permutation-major).
+// ===========================================================================
+
+// per set: ES, NES (naive sequential sum of |ES| in permutation order,
+// statistics.cuh:60-70), NP (statistics.cuh:73-77)
+template <class T>
+__global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
+                            T *__restrict__ out, T *__restrict__ factor) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G) return;
+    const T e0 = es[s];
+    const T ref = __builtin_fabs(e0);
+    T sum = T(0);
+    int64_t c = 0;
+    int64_t p = 0;
+    for (; p + 4 <= P; p += 4) {
+        const T v0 = __builtin_fabs(es[(p + 0) * G + s]);
+        const T v1 = __builtin_fabs(es[(p + 1) * G + s]);
+        const T v2 = __builtin_fabs(es[(p + 2) * G + s]);
+        const T v3 = __builtin_fabs(es[(p + 3) * G + s]);
+        sum += v0; sum += v1; sum += v2; sum += v3;
+        c += (v0 >= ref) + (v1 >= ref) + (v2 >= ref) + (v3 >= ref);
+    }
+    for (; p < P; ++p) {
+        const T v = __builtin_fabs(es[p * G + s]);
+        sum += v;
+        c += v >= ref;
+    }
+    const T f = (T)P / sum;
+    out[s] = e0;
+    out[G + s] = e0 * f;
+    out[2 * G + s] = (T)c / (T)P;
+    factor[s] = f;
+}
+
+// per permutation: extrema of normalised ES over sets (statistics.cuh:85-97)
+template <class T>
+__global__ void k_perm_extrema(const T *__restrict__ es, int64_t P, int G,
+                               const T *__restrict__ factor,
+                               T *__restrict__ maxv, T *__restrict__ minv) {
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (p >= P) return;
+    const int lane = lane_id();
+    T mx = -__builtin_inf(), mn = __builtin_inf();
+    for (int s = lane; s < G; s += kWave) {
+        const T v = es[p * G + s] * factor[s];
+        mx = mx < v ? v : mx;
+        mn = mn > v ? v : mn;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const T omx = __shfl_xor(mx, off, kWave);
+        const T omn = __shfl_xor(mn, off, kWave);
+        mx = mx < omx ? omx : mx;
+        mn = mn > omn ? omn : mn;
+    }
+    if (lane == 0) { maxv[p] = mx; minv[p] = mn; }
+}
+
+// FWER, statistics.cuh:100-110
+template <class T>
+__global__ void k_fwer(T *__restrict__ out, int64_t P, int G,
+                       const T *__restrict__ maxv, const T *__restrict__ minv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G) return;
+    const T score = out[G + s];
+    int64_t c = 0;
+    if (score > T(0)) {
+        for (int64_t p = 0; p < P; ++p) c += maxv[p] >= score;
+    } else {
+        for (int64_t p = 0; p < P; ++p) c += minv[p] <= score;
+    }
+    out[3 * G + s] = (T)c / (T)P;
+}
+
+template <class T>
+__device__ __forceinline__ int lower_bound_t(const T *v, int n, T x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ __forceinline__ int upper_bound_t(const T *v, int n, T x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// FDR histograms.  thr[0..n) are the sorted non-NaN NES.  For every null
+// score z (permutations 1..P-1, normalised as statistics.cuh:90): hub[#thr<=z]
+// and hlb[#thr<z] are counted, plus #z>=0 and #z<=0.
+// LDS=true keeps thresholds and per-workgroup histograms in LDS; LDS=false
+// searches and counts in global memory (large numbers of sets).
+template <class T, bool LDS>
+__global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
+                           const T *__restrict__ factor,
+                           const T *__restrict__ thr, int n,
+                           unsigned long long *__restrict__ hub,
+                           unsigned long long *__restrict__ hlb,
+                           unsigned long long *__restrict__ signs) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const T *t = thr;
+    uint32_t *h1 = nullptr, *h2 = nullptr;
+    if (LDS) {
+        T *tl = reinterpret_cast<T *>(smem_raw);
+        h1 = reinterpret_cast<uint32_t *>(smem_raw + ((sizeof(T) * n + 15) / 16) * 16);
+        h2 = h1 + (n + 1);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = thr[i];
+        for (int i = threadIdx.x; i <= n; i += blockDim.x) { h1[i] = 0; h2[i] = 0; }
+        __syncthreads();
+        t = tl;
+    }
+    unsigned long long npos = 0, nneg = 0;
+    const int64_t total = (P - 1) * (int64_t)G;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = 1 + e / G;
+        const int s = (int)(e % G);
+        const T z = es[p * G + s] * factor[s];
+        if (z != z) continue;
+        const int ub = upper_bound_t(t, n, z);
+        const int lb = lower_bound_t(t, n, z);
+        if (LDS) {
+            atomicAdd(&h1[ub], 1u);
+            atomicAdd(&h2[lb], 1u);
+        } else {
+            atomicAdd(&hub[ub], 1ull);
+            atomicAdd(&hlb[lb], 1ull);
+        }
+        npos += z >= T(0);
+        nneg += z <= T(0);
+    }
+    npos = wave_sum(npos);
+    nneg = wave_sum(nneg);
+    if (lane_id() == 0) {
+        atomicAdd(&signs[0], npos);
+        atomicAdd(&signs[1], nneg);
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int i = threadIdx.x; i <= n; i += blockDim.x) {
+            if (h1[i]) atomicAdd(&hub[i], (unsigned long long)h1[i]);
+            if (h2[i]) atomicAdd(&hlb[i], (unsigned long long)h2[i]);
+        }
+    }
+}
+
+// inclusive prefix sums of both histograms (single workgroup, in place)
+__global__ void k_prefix_u64(unsigned long long *__restrict__ h1,
+                             unsigned long long *__restrict__ h2, int len) {
+    __shared__ unsigned long long carry[2];
+    if (threadIdx.x == 0) { carry[0] = 0; carry[1] = 0; }
+    __syncthreads();
+    for (int base = 0; base < len; base += kWave) {
+        if (threadIdx.x < kWave) {
+            const int i = base + (int)threadIdx.x;
+            unsigned long long a = i < len ? h1[i] : 0ull;
+            unsigned long long b = i < len ? h2[i] : 0ull;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const unsigned long long oa = __shfl_up(a, off, kWave);
+                const unsigned long long ob = __shfl_up(b, off, kWave);
+                if ((int)threadIdx.x >= off) { a += oa; b += ob; }
+            }
+            a += carry[0];
+            b += carry[1];
+            if (i < len) { h1[i] = a; h2[i] = b; }
+            const unsigned long long la = __shfl(a, kWave - 1, kWave);
+            const unsigned long long lb = __shfl(b, kWave - 1, kWave);
+            if (threadIdx.x == 0) { carry[0] = la; carry[1] = lb; }
+        }
+        __syncthreads();
+    }
+}
+
+// FDR per set from the prefix-summed histograms:
+// #z >= t = total - cum_hub[lb(t)], #z <= t = cum_hlb[lb(t)]
+template <class T>
+__global__ void k_fdr_final(T *__restrict__ out, int G,
+                            const T *__restrict__ thr, int n,
+                            const unsigned long long *__restrict__ cub,
+                            const unsigned long long *__restrict__ clb,
+                            const unsigned long long *__restrict__ signs) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G) return;
+    const T nes = out[G + s];
+    if (nes != nes) { out[4 * G + s] = (T)__builtin_nan(""); return; }
+    const unsigned long long total = cub[n];
+    const int lbt = lower_bound_t(thr, n, nes);
+    unsigned long long a, b, dn, dobs;
+    if (nes > T(0)) {
+        a = total - cub[lbt];
+        b = (unsigned long long)(n - lbt);
+        dn = signs[0];
+        dobs = (unsigned long long)(n - lower_bound_t(thr, n, T(0)));
+    } else {
+        a = clb[lbt];
+        b = (unsigned long long)upper_bound_t(thr, n, nes);
+        dn = signs[1];
+        dobs = (unsigned long long)upper_bound_t(thr, n, T(0));
+    }
+    const double num = dn ? (double)a / (double)dn : 0.0;
+    const double den = (double)b / (double)dobs;
+    const double q = num / den;
+    out[4 * G + s] = (T)(q < 1.0 ? q : 1.0);
+}
+
+// [P][G] -> [G][P]
+template <class T>
+__global__ void k_transpose(const T *__restrict__ in, int64_t P, int G,
+                            T *__restrict__ out) {
+    __shared__ T tile[32][33];
+    const int64_t p0 = (int64_t)blockIdx.y * 32;
+    const int s0 = blockIdx.x * 32;
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int64_t p = p0 + i;
+        const int s = s0 + threadIdx.x;
+        if (p < P && s < G) tile[i][threadIdx.x] = in[p * G + s];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int s = s0 + i;
+        const int64_t p = p0 + threadIdx.x;
+        if (p < P && s < G) out[(int64_t)s * P + p] = tile[threadIdx.x][i];
+    }
+}
+
+}  // namespace gsea

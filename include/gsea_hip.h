@@ -1,0 +1,178 @@
+/*
+ * gsea_hip.h -- C ABI of the MI355X (gfx950) permutation-GSEA engine,
+ * libgsea_hip.so.  Plain pointers and sizes only.
+ *
+ * The reference binds its engine below Rcpp through the C++ proxy ABI of
+ * cudaGSEA/src/cudaGSEA_wrapper.cuh:15-61 (implemented in
+ * cudaGSEA/src/cudaGSEA_wrapper.cu).  That ABI is provided unchanged by
+ * include/cudaGSEA_wrapper.cuh on top of the functions below; this header is
+ * the thin C layer underneath it, callable from C, ctypes, cgo or JNI.
+ *
+ * Conventions
+ *   - exprs is sample-major [S][N]: exprs[s*N + g], the layout the R path
+ *     hands over (cudaGSEA_wrapper.cpp:231/:253, cudaGSEA_wrapper.cu:229-236).
+ *   - labels: S bytes, 0 = class A, 1 = class B (cudaGSEA_wrapper.cu:239-244).
+ *   - gene sets: CSR over gene indices, set_offsets[G+1], set_genes[...];
+ *     within a set gene indices must be strictly increasing and < N (this is
+ *     what gsea_map_sets produces from symbols).
+ *   - es_out (optional) is [G][P] set-major, column 0 the observed labels:
+ *     the reference's global_result / dump layout (copy_strided.cuh:9-36).
+ *   - stats_out is [5][G]: ES, NES, NP, FWER, FDR (statistics.cuh:51-53; the
+ *     reference leaves FDR NaN, statistics.cuh:112-114, this engine fills it).
+ *   - Return value: 0 on success, otherwise an error code below; the message
+ *     is available from gsea_last_error() (per thread).  Nothing calls exit().
+ */
+#ifndef GSEA_HIP_H
+#define GSEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes: 1-22 keep the numbering of cudaGSEA/src/include/error_codes.cuh */
+#define GSEA_OK                         0
+#define GSEA_ERR_UNKNOWN_METRIC         1   /* CUDA_GSEA_NO_KNOWN_METRIC_SPECIFIED_ERROR */
+#define GSEA_ERR_CANNOT_OPEN_FILE       2   /* CUDA_GSEA_CANNOT_OPEN_FILE_ERROR */
+#define GSEA_ERR_NO_PATHS               10  /* CUDA_GSEA_NO_PATHS_SPECIFIED_ERROR */
+#define GSEA_ERR_EMPTY_DUMP_FILE_NAME   19  /* CUDA_GSEA_EMPTY_DUMP_FILE_NAME_ERROR */
+#define GSEA_ERR_INVALID_ARGUMENT       30  /* bad sizes, labels or gene sets */
+#define GSEA_ERR_UNSUPPORTED_SIZE       31  /* exceeds a documented engine limit */
+#define GSEA_ERR_HIP                    32  /* HIP runtime error */
+#define GSEA_ERR_OUT_OF_MEMORY          33
+
+/* flags */
+#define GSEA_FLAG_ASCENDING   1   /* reference sort_direction == false */
+#define GSEA_FLAG_SWAP_LABELS 2   /* reference swap_labels == true */
+
+/* engine limits */
+#define GSEA_MAX_GENES   32768
+#define GSEA_MAX_SAMPLES 65535
+
+/* per-stage device times of the last call, milliseconds (HIP events) */
+typedef struct gsea_timings {
+    double ms_lists;      /* permutation class lists */
+    double ms_metric;     /* local ranking metric */
+    double ms_rank_es;    /* argsort + enrichment scores */
+    double ms_stats;      /* NES, NP, FWER, FDR */
+    double ms_transpose;  /* [P][G] -> [G][P] when es_out is requested */
+    double ms_total;      /* first launch to last completion */
+    int64_t perm_batches; /* permutation batches used */
+} gsea_timings;
+
+/* metric name <-> id; ids follow the dispatch order of
+ * cudaGSEA/src/include/correlate_genes.cuh:166-295 */
+int         gsea_metric_id(const char *name);          /* -1 if unknown */
+const char *gsea_metric_name(int id);                  /* "" if unknown */
+const char *gsea_last_error(void);
+const char *gsea_version(void);
+
+/* device management; replaces cudaGSEA_wrapper.cu:87-138
+ * (listCudaDevices_proxy / getCudaDevice_proxy / setCudaDevice_proxy) */
+int gsea_device_count(int *count);
+int gsea_get_device(int *device);
+int gsea_set_device(int device);
+int gsea_device_name(int device, char *name, size_t name_len,
+                     size_t *total_bytes);
+
+/* Full path with host buffers: replaces gsea_proxy<...>'s device work,
+ * cudaGSEA_wrapper.cu:173-401 (correllate_genes_gpu, pathway_sort_gpu,
+ * compute_scores_gpu, final_statistics).  fp32: gsea_single_proxy's types
+ * (cudaGSEA_wrapper.cu:403-419); fp64: gsea_double_proxy (:421-437). */
+int gsea_run_f32(const float *exprs, int64_t num_genes, int64_t num_samples,
+                 const uint8_t *labels, const int32_t *set_offsets,
+                 const int32_t *set_genes, int64_t num_sets,
+                 int64_t num_perms, int metric_id, int flags,
+                 float *es_out, float *stats_out, gsea_timings *timings);
+int gsea_run_f64(const double *exprs, int64_t num_genes, int64_t num_samples,
+                 const uint8_t *labels, const int32_t *set_offsets,
+                 const int32_t *set_genes, int64_t num_sets,
+                 int64_t num_perms, int metric_id, int flags,
+                 double *es_out, double *stats_out, gsea_timings *timings);
+
+/* Same path with every array already in device memory (labels stay on the
+ * host: S bytes, read synchronously).  d_es_out may be NULL.  Work is queued
+ * on `stream` (a hipStream_t, NULL = default stream); the call returns after
+ * the statistics are complete. */
+int gsea_run_device_f32(const float *d_exprs, int64_t num_genes,
+                        int64_t num_samples, const uint8_t *labels,
+                        const int32_t *d_set_offsets,
+                        const int32_t *d_set_genes, int64_t num_sets,
+                        int64_t num_perms, int metric_id, int flags,
+                        float *d_es_out, float *d_stats_out, void *stream,
+                        gsea_timings *timings);
+int gsea_run_device_f64(const double *d_exprs, int64_t num_genes,
+                        int64_t num_samples, const uint8_t *labels,
+                        const int32_t *d_set_offsets,
+                        const int32_t *d_set_genes, int64_t num_sets,
+                        int64_t num_perms, int metric_id, int flags,
+                        double *d_es_out, double *d_stats_out, void *stream,
+                        gsea_timings *timings);
+
+/* Stage entry points (device pointers), for tests and callers that hold
+ * intermediate results.  Replace correllate_genes_gpu
+ * (correlate_genes.cuh:121-300) and pathway_sort_gpu + compute_scores_gpu
+ * (batched_sort.cuh:85-269, enrichment_scores.cuh:208-256).
+ *   gsea_metric_*: r[P][N] for global permutations shift..shift+P-1.
+ *   gsea_rank_es_*: es[P][G] (permutation-major) from r[P][N]. */
+int gsea_metric_f32(const float *d_exprs, int64_t num_genes,
+                    int64_t num_samples, const uint8_t *labels,
+                    int64_t num_perms, int64_t shift, int metric_id, int flags,
+                    float *d_r, void *stream);
+int gsea_metric_f64(const double *d_exprs, int64_t num_genes,
+                    int64_t num_samples, const uint8_t *labels,
+                    int64_t num_perms, int64_t shift, int metric_id, int flags,
+                    double *d_r, void *stream);
+int gsea_rank_es_f32(const float *d_r, int64_t num_genes, int64_t num_perms,
+                     const int32_t *d_set_offsets, const int32_t *d_set_genes,
+                     int64_t num_sets, int flags, float *d_es, void *stream);
+int gsea_rank_es_f64(const double *d_r, int64_t num_genes, int64_t num_perms,
+                     const int32_t *d_set_offsets, const int32_t *d_set_genes,
+                     int64_t num_sets, int flags, double *d_es, void *stream);
+
+/* Stage (b) alone: per permutation, the gene index at each rank under the
+ * stated order (descending metric unless GSEA_FLAG_ASCENDING; ties by
+ * ascending gene index; -0 below +0; NaN above +inf).  Replaces
+ * pathway_sort_gpu's output Index (batched_sort.cuh:85-269). */
+int gsea_argsort_f32(const float *d_keys, int64_t num_genes, int64_t num_perms,
+                     int flags, uint32_t *d_index, void *stream);
+int gsea_argsort_f64(const double *d_keys, int64_t num_genes,
+                     int64_t num_perms, int flags, uint32_t *d_index,
+                     void *stream);
+
+/* Gene-set symbols -> CSR of gene indices with create_bitmaps semantics
+ * (cudaGSEA/src/include/create_bitmaps.cuh:13-65): gene g belongs to set s
+ * when gene_symbols[g] equals one of the set's symbols exactly; symbols
+ * missing from the data are ignored; duplicates collapse.  set_symbols of set
+ * s are set_symbols[set_symbol_offsets[s] .. set_symbol_offsets[s+1]).
+ * set_offsets_out gets G+1 entries; set_genes_out up to `capacity` entries;
+ * *total_out receives the number needed (call with capacity 0 to size). */
+int gsea_map_sets(const char *const *gene_symbols, int64_t num_genes,
+                  const char *const *set_symbols,
+                  const int64_t *set_symbol_offsets, int64_t num_sets,
+                  int32_t *set_offsets_out, int32_t *set_genes_out,
+                  int64_t capacity, int64_t *total_out);
+
+/* Raw ES dump, dump_binary.cuh:10-30 with the naming of
+ * cudaGSEA_wrapper.cu:384-390: writes <prefix>_<G>_<P>_<bits>.es, native
+ * endian, no header, row-major [G][P]. */
+int gsea_dump_es(const char *prefix, const void *es, int64_t num_sets,
+                 int64_t num_perms, int bits);
+
+/* Whole R-level call with symbols (what gsea_single_proxy /
+ * gsea_double_proxy do, cudaGSEA_wrapper.cu:173-437): map sets, run,
+ * dump when dump_prefix is non-empty, return [5][G] statistics. */
+int gsea_proxy_run(int double_precision, const void *exprs, int64_t num_genes,
+                   int64_t num_samples, const char *const *gene_symbols,
+                   const uint8_t *labels, const char *const *set_symbols,
+                   const int64_t *set_symbol_offsets, int64_t num_sets,
+                   const char *metric, int64_t num_perms, int sort_descending,
+                   int swap_labels, const char *dump_prefix, void *stats_out,
+                   gsea_timings *timings);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""C ABI checks that need no GPU: the library loads, exports what
+include/gsea_hip.h declares, and the host-side logic (metric names, set
+mapping, dump files, argument errors, GSEA() input checks) behaves."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "gsea_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gsea_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import cudagsea_amd
+    L = cudagsea_amd.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    # and the ctypes table covers them all
+    from cudagsea_amd._lib import SIGNATURES
+    assert set(names) <= set(SIGNATURES), set(names) - set(SIGNATURES)
+
+
+def test_proxy_abi_symbols_exported():
+    # C++ proxy ABI of cudaGSEA_wrapper.cuh, looked up by mangled name
+    import subprocess
+    lib = os.path.join(ROOT, "cudagsea_amd", "libgsea_hip.so")
+    out = subprocess.run(["nm", "-DC", lib], stdout=subprocess.PIPE, text=True).stdout
+    for sym in ("gsea_single_proxy(", "gsea_double_proxy(", "listCudaDevices_proxy",
+                "getCudaDevice_proxy", "setCudaDevice_proxy",
+                "loadLabelsFromCLS_proxy", "loadGeneSetsFromGMT_proxy",
+                "loadExpressionDataFromGCT_proxy"):
+        assert sym in out, sym
+
+
+def test_metric_names_roundtrip():
+    import cudagsea_amd as cg
+    L = cg.lib()
+    for i, m in enumerate(cg.METRICS):
+        assert L.gsea_metric_id(m.encode()) == i
+        assert L.gsea_metric_name(i).decode() == m
+    assert L.gsea_metric_id(b"") == -1
+    assert L.gsea_metric_id(b"pearson") == -1
+    assert L.gsea_metric_name(14) == b""
+
+
+def test_map_sets_semantics():
+    import cudagsea_amd as cg
+    genes = ["TP53", "BRCA1", "EGFR", "TP53", "MYC"]
+    sets = [["TP53", "NOPE"], ["EGFR", "EGFR", "MYC"], ["ABSENT"], ["MYC", "BRCA1"]]
+    off, g = cg.map_sets(genes, sets)
+    got = [g[off[i]:off[i + 1]].tolist() for i in range(len(sets))]
+    # brute force of create_bitmaps.cuh:38-60
+    want = [[k for k, sym in enumerate(genes) if sym in set(s)] for s in sets]
+    assert got == want
+
+
+def test_dump_naming_and_content(tmp_path):
+    import cudagsea_amd as cg
+    es = np.arange(12, dtype=np.float32).reshape(3, 4)
+    prefix = str(tmp_path / "run")
+    cg._lib.check(cg.lib().gsea_dump_es(prefix.encode(), es.ctypes.data_as(ctypes.c_void_p),
+                                        3, 4, 32))
+    path = prefix + "_3_4_32.es"
+    assert os.path.getsize(path) == 48
+    assert (cg.read_dump(path) == es).all()
+    rc = cg.lib().gsea_dump_es(b"", es.ctypes.data_as(ctypes.c_void_p), 3, 4, 32)
+    assert rc == 19                      # CUDA_GSEA_EMPTY_DUMP_FILE_NAME_ERROR
+
+
+def test_argument_errors_before_any_device_call():
+    import cudagsea_amd as cg
+    X = np.ones((4, 10), dtype=np.float32)
+    lab = np.array([0, 0, 1, 1], dtype=np.uint8)
+    off = np.array([0, 2], dtype=np.int32)
+    genes = np.array([1, 3], dtype=np.int32)
+    with pytest.raises(cg.GseaError) as e:
+        cg.run(X, lab, off, genes, 8, 99)
+    assert e.value.code == 1             # no known metric
+    with pytest.raises(cg.GseaError) as e:
+        cg.run(X, lab, off, np.array([3, 1], dtype=np.int32), 8, 0)
+    assert e.value.code == 30            # unsorted set
+    with pytest.raises(cg.GseaError) as e:
+        cg.run(X, lab, off, np.array([1, 10], dtype=np.int32), 8, 0)
+    assert e.value.code == 30            # gene out of range
+    with pytest.raises(cg.GseaError) as e:
+        cg.run(np.ones((4, 40000), dtype=np.float32), lab, off, genes, 8, 0)
+    assert e.value.code == 31            # above GSEA_MAX_GENES
+
+
+def _frame():
+    import pandas as pd
+    rng = np.random.default_rng(0)
+    return pd.DataFrame(rng.random((6, 4)) + 1, index=["G%d" % i for i in range(6)],
+                        columns=["s%d" % i for i in range(4)])
+
+
+def test_gsea_input_checks_return_false(capsys):
+    import cudagsea_amd as cg
+    df = _frame()
+    sets = {"a": ["G1", "G2"]}
+    assert cg.GSEA(df, [0, 1, 0], sets, 10, "onepass_t_test") is False
+    assert cg.GSEA(df, [0, 1, 2, 1], sets, 10, "onepass_t_test") is False
+    bad = df.copy()
+    bad.iloc[2, 2] = np.nan
+    assert cg.GSEA(bad, [0, 1, 0, 1], sets, 10, "onepass_t_test") is False
+    assert cg.GSEA(df, [0, 1, 0, 1], {}, 10, "onepass_t_test") is False
+    assert cg.GSEA(df, [0, 1, 0, 1], {"a": []}, 10, "onepass_t_test") is False
+    assert capsys.readouterr().out.count("INVALID DATA") == 5
+
+
+def test_gsea_unknown_metric_raises_instead_of_exit():
+    import cudagsea_amd as cg
+    df = _frame()
+    with pytest.raises(cg.GseaError) as e:
+        cg.GSEA(df, [0, 1, 0, 1], {"a": ["G1"]}, 10, "")
+    assert e.value.code == 1
