@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgsea_hip.so")
 
-SOURCES = ["gsea_engine.hip", "gsea_proxy.cpp", "gsea_io.cpp"]
+SOURCES = ["gsea_engine.hip", "gsea_surrogate.hip", "gsea_proxy.cpp", "gsea_io.cpp"]
 
 # IEEE arithmetic in source order: no contraction, no fast-math; packed FP32
 # instructions disabled for the device code.

@@ -94,6 +94,12 @@ static int device_state(DeviceState **out) {
         int v = 0;
         HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         d.lds_max = v;
+        // the opt-in limit (dynamic LDS above the default) where reported
+        int optin = 0;
+        if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) ==
+                hipSuccess && optin > d.lds_max)
+            d.lds_max = optin;
+        (void)hipGetLastError();
         HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
         d.cu_count = v;
         d.init = true;
@@ -175,7 +181,8 @@ static size_t align16(size_t x) { return (x + 15) / 16 * 16; }
 
 struct Labels {
     std::vector<uint8_t> lab;
-    int nA = 0, nB = 0;
+    int nA = 0, nB = 0;          // divisors: label-0 / label-1 counts
+    int lenA = 0;                // samples with label 0 after any swap
     int offB = 0, lstride = 0;
 };
 
@@ -196,6 +203,7 @@ static int prepare_labels(const uint8_t *labels, int64_t S, int flags,
     for (auto v : L->lab) n0 += v == 0;
     L->nA = nA;
     L->nB = nB;
+    L->lenA = n0;
     L->offB = (n0 + 7) / 8 * 8;
     L->lstride = L->offB + ((int)S - n0 + 7) / 8 * 8;
     return GSEA_OK;
@@ -244,6 +252,7 @@ static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
     MetricArgs<T> a;
     a.X = dX; a.N = (int)N; a.S = (int)S; a.lists = lists;
     a.lstride = L.lstride; a.nA = L.nA; a.nB = L.nB; a.offB = L.offB;
+    a.lenA = L.lenA;
     a.Pb = (int)Pb; a.metric = metric; a.r = r;
     bool xlds = false;
     const int gt = metric_tile<T>(S, &xlds);

@@ -74,7 +74,8 @@ struct MetricArgs {
     const T *X;            // [S][N] sample-major
     int N, S;
     const uint16_t *lists; // [Pb][lstride]
-    int lstride, nA, nB, offB;
+    int lstride, nA, nB, offB;   // nA, nB: divisors (label 0 / 1 counts)
+    int lenA;                    // samples with label 0 (class-A list)
     int Pb, ppb;
     int metric;            // 0..13, selects the combiner
     T *r;                  // [Pb][N]
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
     }
 
     const T nA = (T)a.nA, nB = (T)a.nB;
-    const int nRest = a.S - a.nA;
+    const int nRest = a.S - a.lenA;   // samples whose label is not 0
     const int p_begin = blockIdx.y * a.ppb;
     const int p_end = min(a.Pb, p_begin + a.ppb);
     const int slot = wave * PPW + grp;
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             T sA[PL], sB[PL], qA[PL], qB[PL];
 #pragma unroll
             for (int k = 0; k < PL; ++k) sA[k] = sB[k] = qA[k] = qB[k] = T(0);
-            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+            walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
                 const T v = X_at(s);
                 sA[k] = sA[k] + v;
                 if (KIND == kOnepass) qA[k] = qA[k] + v * v;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
                 bool bA[PL], bB[PL];
 #pragma unroll
                 for (int k = 0; k < PL; ++k) bA[k] = bB[k] = false;
-                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
                     const T v = X_at(s);
                     bA[k] |= !(v - v == T(0));
                 });
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
                 }
             } else {
                 // functors.cuh:270-290
-                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
                     const T d = X_at(s) - mA[k];
                     qA[k] = qA[k] + d * d;
                 });
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             T sA[PL], sB[PL], cA[PL], cB[PL];
 #pragma unroll
             for (int k = 0; k < PL; ++k) sA[k] = sB[k] = cA[k] = cB[k] = T(0);
-            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+            walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
                 const T alpha = X_at(s) - cA[k];
                 const T beta = sA[k] + alpha;
                 cA[k] = (beta - sA[k]) - alpha;
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
                 T uA[PL], uB[PL], vA[PL], vB[PL];
 #pragma unroll
                 for (int k = 0; k < PL; ++k) uA[k] = uB[k] = vA[k] = vB[k] = T(0);
-                walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int) {
+                walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
                     const T d = X_at(s) - mA[k];
                     uA[k] = uA[k] + d;
                     vA[k] = vA[k] + d * d;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             T mA[PL], mB[PL], vA[PL], vB[PL];
 #pragma unroll
             for (int k = 0; k < PL; ++k) mA[k] = mB[k] = vA[k] = vB[k] = T(0);
-            walk_lists<PL>(L, 0, a.nA, [&](int k, int s, int i) {
+            walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int i) {
                 const T v = X_at(s);
                 const T delta = v - mA[k];
                 mA[k] = mA[k] + delta / (T)(i + 1);

@@ -172,6 +172,38 @@ int gsea_proxy_run(int double_precision, const void *exprs, int64_t num_genes,
                    int swap_labels, const char *dump_prefix, void *stats_out,
                    gsea_timings *timings);
 
+/* ---- learned permutation allocation (no counterpart in the reference) ----
+ * A 16 -> 64 -> 64 -> 4 MLP with GELU predicts a set's null ES moments
+ * (mean/std of the positive and of the negative null ES) from 16 set
+ * features; trained in fp32 by these kernels, inferred in fp32 or bf16.
+ * Parameters packed: W1[64][16] b1[64] W2[64][64] b2[64] W3[4][64] b3[4]. */
+#define GSEA_MLP_IN 16
+#define GSEA_MLP_HIDDEN 64
+#define GSEA_MLP_OUT 4
+#define GSEA_MLP_NUM_PARAMS 5508
+
+int gsea_mlp_num_params(void);
+/* y[B][4] = mlp(x[B][16]); bf16 != 0 rounds weights and activations to bf16 */
+int gsea_mlp_forward(const float *d_params, const float *d_x, int64_t batch,
+                     int bf16, float *d_y, void *stream);
+/* gradient of mean((mlp(x) - t)^2) over batch x 4 outputs; *loss (host) */
+int gsea_mlp_grad(const float *d_params, const float *d_x, const float *d_t,
+                  int64_t batch, float *d_grad, float *loss, void *stream);
+/* one Adam step (beta 0.9/0.999, eps 1e-8, decoupled weight decay added to
+ * the gradient); step counts from 1 */
+int gsea_mlp_adam(float *d_params, float *d_m, float *d_v, const float *d_grad,
+                  int64_t step, float lr, float weight_decay, void *stream);
+/* features[G][16] of every set from the observed metric r0[N] and observed
+ * ranks pos0[N]; glob[4] receives summaries of r0 */
+int gsea_set_features(const float *d_r0, const int32_t *d_pos0, int64_t num_genes,
+                      const int32_t *d_set_offsets, const int32_t *d_set_genes,
+                      int64_t num_sets, float *d_glob, float *d_feat,
+                      void *stream);
+/* out[G][5]: mean/std of positive null ES, mean/std of |negative null ES|,
+ * fraction positive, over permutations 1..P-1 of es[P][G] */
+int gsea_null_moments(const float *d_es, int64_t num_perms, int64_t num_sets,
+                      float *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
