@@ -75,6 +75,12 @@ SIGNATURES = {
     "gsea_map_sets": (_I, [_P, _I64, _P, _P, _I64, _P, _P, _I64,
                            ctypes.POINTER(_I64)]),
     "gsea_dump_es": (_I, [ctypes.c_char_p, _P, _I64, _I64, _I]),
+    "gsea_mlp_num_params": (_I, []),
+    "gsea_mlp_forward": (_I, [_P, _P, _I64, _I, _P, _P]),
+    "gsea_mlp_grad": (_I, [_P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float), _P]),
+    "gsea_mlp_adam": (_I, [_P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, _P]),
+    "gsea_set_features": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P]),
+    "gsea_null_moments": (_I, [_P, _I64, _I64, _P, _P]),
     "gsea_proxy_run": (_I, [_I, _P, _I64, _I64, _P, _P, _P, _P, _I64,
                             ctypes.c_char_p, _I64, _I, _I, ctypes.c_char_p, _P,
                             _P]),

@@ -3,7 +3,7 @@
 //
 // Pipeline per permutation batch (the reference's gsea_proxy loop,
 // cudaGSEA_wrapper.cu:295-372, without its per-tile host work):
-//   k_class_lists -> k_metric -> k_rank_es          (ES[P][G] in HBM)
+//   k_class_lists -> k_metric -> k_rank -> k_es     (ES[P][G] in HBM)
 // then once: k_set_stats -> k_perm_extrema -> k_fwer -> FDR kernels, and
 // k_transpose when the caller wants the [G][P] matrix.
 //
@@ -65,7 +65,7 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kNumBufs
+    kBufStats, kBufEsOut, kBufTier, kBufTierGm, kBufPos, kBufDecay, kNumBufs
 };
 
 struct DeviceState {
@@ -276,10 +276,11 @@ static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
     return GSEA_OK;
 }
 
+// launches with more than the default 64 KiB of dynamic LDS opt in first
 template <class T, bool ALDS, int MAXCH>
 static hipError_t launch_rank_mc(const RankArgs<T> &a, size_t lds, int grid,
                                  hipStream_t st) {
-    auto kern = k_rank_es<T, ALDS, MAXCH>;
+    auto kern = k_rank<T, ALDS, MAXCH>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -288,21 +289,34 @@ static hipError_t launch_rank_mc(const RankArgs<T> &a, size_t lds, int grid,
     return hipGetLastError();
 }
 
-template <class T, bool ALDS>
-static hipError_t launch_rank_alds(const RankArgs<T> &a, int maxch, size_t lds,
-                                   int grid, hipStream_t st) {
+template <class T, bool ALDS, int W>
+static hipError_t launch_es_w(const RankArgs<T> &a, size_t lds, int grid,
+                              hipStream_t st) {
+    auto kern = k_es<T, ALDS, W>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kRankThreads), lds, st, a);
+    return hipGetLastError();
+}
+
+// MAXCH chunks of 64 keys per wavefront; k_es lanes own (MAXCH+1)/2 words
+template <class T, bool RANK, bool ALDS>
+static hipError_t launch_by_maxch(const RankArgs<T> &a, int maxch, size_t lds,
+                                  int grid, hipStream_t st) {
+#define GSEA_CASE(C)                                                         \
+    case C:                                                                  \
+        return RANK ? launch_rank_mc<T, ALDS, C>(a, lds, grid, st)           \
+                    : launch_es_w<T, ALDS, (C + 1) / 2>(a, lds, grid, st);
     switch (maxch) {
-    case 1: return launch_rank_mc<T, ALDS, 1>(a, lds, grid, st);
-    case 2: return launch_rank_mc<T, ALDS, 2>(a, lds, grid, st);
-    case 4: return launch_rank_mc<T, ALDS, 4>(a, lds, grid, st);
-    case 8: return launch_rank_mc<T, ALDS, 8>(a, lds, grid, st);
-    case 12: return launch_rank_mc<T, ALDS, 12>(a, lds, grid, st);
-    case 16: return launch_rank_mc<T, ALDS, 16>(a, lds, grid, st);
-    case 20: return launch_rank_mc<T, ALDS, 20>(a, lds, grid, st);
-    case 24: return launch_rank_mc<T, ALDS, 24>(a, lds, grid, st);
-    case 28: return launch_rank_mc<T, ALDS, 28>(a, lds, grid, st);
-    default: return launch_rank_mc<T, ALDS, 32>(a, lds, grid, st);
+        GSEA_CASE(1) GSEA_CASE(2) GSEA_CASE(4) GSEA_CASE(8) GSEA_CASE(12)
+        GSEA_CASE(16) GSEA_CASE(20) GSEA_CASE(24) GSEA_CASE(28)
+    default:
+        return RANK ? launch_rank_mc<T, ALDS, 32>(a, lds, grid, st)
+                    : launch_es_w<T, ALDS, 16>(a, lds, grid, st);
     }
+#undef GSEA_CASE
 }
 
 static int pick_maxch(int64_t N) {
@@ -312,52 +326,133 @@ static int pick_maxch(int64_t N) {
     return 32;
 }
 
-// LDS layout of k_rank_es and the number of wavefronts that score sets
-struct RankPlan {
+// LDS layouts.  k_rank: [keys (fp32 only)] | ids | 16 x 256 counters | misc.
+// k_es: [|metric| by rank (fp32 when it fits)] | pos | per-wavefront bitmaps
+// of the wavefronts that score sets | misc.
+struct LdsPlan {
     bool alds = false;
-    int maxch = 1;
     size_t lds = 0;
-    int es_waves = 16;
-    int nw = 0;
+    int waves = 16;
     size_t off_ids = 0, off_cnt = 0, off_misc = 0;
 };
+
+struct RankPlan {
+    int maxch = 1;
+    int nw = 0;
+    LdsPlan rank, es;
+};
+
+static bool fit(size_t region0, size_t npad, size_t region2, int lds_max, LdsPlan *o) {
+    size_t off = align16(region0);
+    o->off_ids = off;
+    off += align16(npad * 2);
+    o->off_cnt = off;
+    off += align16(region2);
+    o->off_misc = off;
+    off += 64 * 4;
+    o->lds = off;
+    return off <= (size_t)lds_max;
+}
 
 template <class T>
 static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     pl->maxch = pick_maxch(N);
     const size_t npad = (size_t)kRankThreads * pl->maxch;
     pl->nw = (int)((npad + 31) / 32);
+    const size_t counters = (size_t)kRankWaves * 256 * 4;
+    // k_rank keeps fp32 keys in LDS; fp64 keys are recomputed from r
+    pl->rank.alds = sizeof(T) == 4 && fit(npad * 4, npad, counters, d->lds_max, &pl->rank);
+    if (!pl->rank.alds && !fit(0, npad, counters, d->lds_max, &pl->rank))
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
     for (int alds = (sizeof(T) == 4) ? 1 : 0; alds >= 0; --alds) {
         for (int ew = kRankWaves; ew >= 1; --ew) {
-            size_t off = 0;
-            size_t region0 = alds ? align16(npad * 4) : 0;
-            off += region0;
-            const size_t off_ids = off;
-            off += align16(npad * 2);
-            const size_t off_cnt = off;
-            off += align16(std::max<size_t>((size_t)kRankWaves * 256 * 4,
-                                            (size_t)ew * pl->nw * 4));
-            const size_t off_misc = off;
-            off += 64 * 4;
-            if (off <= (size_t)d->lds_max) {
-                pl->alds = alds != 0;
-                pl->lds = off;
-                pl->es_waves = ew;
-                pl->off_ids = off_ids;
-                pl->off_cnt = off_cnt;
-                pl->off_misc = off_misc;
+            if (fit(alds ? npad * 4 : 0, npad, (size_t)ew * pl->nw * 4, d->lds_max, &pl->es)) {
+                pl->es.alds = alds != 0;
+                pl->es.waves = ew;
                 return GSEA_OK;
             }
         }
     }
-    return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
+    return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the ES kernel");
+}
+
+// Work split of the ES phase: sets above kSmallSet genes are scored by one
+// wavefront each (largest first), the others by one lane each in chunks of
+// 64 sets of one size class (<= 8, <= 16, <= 32 genes).
+struct SetTiers {
+    int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, stride = 0;
+    const int32_t *large_ids = nullptr, *small_ids = nullptr, *small_k = nullptr,
+                  *small_gm = nullptr;
+    const void *decay = nullptr;   // [G] of T
+};
+
+template <class T>
+static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                       int64_t N, DeviceState *d, hipStream_t st, SetTiers *t) {
+    std::vector<int32_t> off((size_t)G + 1);
+    HIP_TRY(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (G + 1),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int32_t> large, cls[3];
+    for (int64_t s = 0; s < G; ++s) {
+        const int k = off[(size_t)s + 1] - off[(size_t)s];
+        if (k < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
+        if (k > kSmallSet) large.push_back((int32_t)s);
+        else cls[k <= 8 ? 0 : k <= 16 ? 1 : 2].push_back((int32_t)s);
+    }
+    std::stable_sort(large.begin(), large.end(), [&](int32_t x, int32_t y) {
+        return off[x + 1] - off[x] > off[y + 1] - off[y];
+    });
+    std::vector<int32_t> sid, sk;
+    int ends[3];
+    for (int c = 0; c < 3; ++c) {
+        for (int32_t s : cls[c]) { sid.push_back(s); sk.push_back(off[s + 1] - off[s]); }
+        while (sid.size() % kWave) { sid.push_back(-1); sk.push_back(0); }
+        ends[c] = (int)(sid.size() / kWave);
+    }
+    t->n_large = (int)large.size();
+    t->n_chunks = ends[2];
+    t->chunk_end8 = ends[0];
+    t->chunk_end16 = ends[1];
+    t->stride = (int)sid.size();
+    const size_t nl = large.size(), ns = sid.size();
+    std::vector<int32_t> host(nl + 2 * ns);
+    std::copy(large.begin(), large.end(), host.begin());
+    std::copy(sid.begin(), sid.end(), host.begin() + nl);
+    std::copy(sk.begin(), sk.end(), host.begin() + nl + ns);
+    // per-set decay of the running sum, enrichment_scores.cuh:311
+    std::vector<T> decay((size_t)G);
+    for (int64_t s = 0; s < G; ++s)
+        decay[(size_t)s] = (T)(-1.0 / (double)(N - (off[(size_t)s + 1] - off[(size_t)s])));
+    void *pt, *pg, *pd;
+    int rc = ensure(d, kBufTier, sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
+    if (!rc) rc = ensure(d, kBufDecay, sizeof(T) * std::max<int64_t>(G, 1), &pd);
+    if (!rc) rc = ensure(d, kBufTierGm, sizeof(int32_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
+    if (rc) return rc;
+    if (!host.empty())
+        HIP_TRY(hipMemcpyAsync(pt, host.data(), sizeof(int32_t) * host.size(),
+                               hipMemcpyHostToDevice, st));
+    if (G) HIP_TRY(hipMemcpyAsync(pd, decay.data(), sizeof(T) * G, hipMemcpyHostToDevice, st));
+    t->decay = pd;
+    t->large_ids = (const int32_t *)pt;
+    t->small_ids = (const int32_t *)pt + nl;
+    t->small_k = (const int32_t *)pt + nl + ns;
+    t->small_gm = (const int32_t *)pg;
+    if (ns) {
+        hipLaunchKernelGGL(k_small_gm, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
+                           d_off, d_genes, t->small_ids, (int)ns, (int32_t *)pg);
+        HIP_TRY(hipGetLastError());
+    }
+    // the host vectors above must outlive the asynchronous copy
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
 }
 
 template <class T>
 static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
                        const int32_t *d_off, const int32_t *d_genes, int64_t G,
                        int flags, T *es_pg, DeviceState *d, hipStream_t st,
-                       uint32_t *idx_out = nullptr) {
+                       const SetTiers *tiers, uint32_t *idx_out = nullptr) {
     RankPlan pl;
     int rc = plan_rank<T>(N, d, &pl);
     if (rc) return rc;
@@ -365,20 +460,43 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
     a.r = r; a.N = (int)N; a.Pb = (int)Pb; a.perm_base = perm_base;
     a.G = (int)G; a.set_off = d_off; a.set_genes = d_genes; a.es = es_pg;
     a.asc = (flags & GSEA_FLAG_ASCENDING) ? 1 : 0;
-    a.es_waves = pl.es_waves; a.nw = pl.nw;
-    a.off_ids = pl.off_ids; a.off_cnt = pl.off_cnt; a.off_misc = pl.off_misc;
+    a.nw = pl.nw;
     a.a_glob = nullptr;
+    a.pos_out = nullptr;
     a.idx_out = idx_out;
-    if (!pl.alds && G > 0) {
-        void *p = nullptr;
-        rc = ensure(d, kBufAGlob, (size_t)Pb * N * sizeof(T), &p);
+    a.n_large = tiers ? tiers->n_large : 0;
+    a.n_items = tiers ? tiers->n_large + tiers->n_chunks : 0;
+    a.large_ids = tiers ? tiers->large_ids : nullptr;
+    a.small_ids = tiers ? tiers->small_ids : nullptr;
+    a.small_k = tiers ? tiers->small_k : nullptr;
+    a.small_gm = tiers ? tiers->small_gm : nullptr;
+    a.small_stride = tiers ? tiers->stride : 0;
+    a.chunk_end8 = tiers ? tiers->chunk_end8 : 0;
+    a.chunk_end16 = tiers ? tiers->chunk_end16 : 0;
+    a.decay = tiers ? (const T *)tiers->decay : nullptr;
+    if (G > 0) {
+        // rank rows handed from k_rank to k_es: |metric| by rank, rank by gene
+        void *pa = nullptr, *pp = nullptr;
+        rc = ensure(d, kBufAGlob, (size_t)Pb * N * sizeof(T), &pa);
+        if (!rc) rc = ensure(d, kBufPos, (size_t)Pb * N * sizeof(uint16_t), &pp);
         if (rc) return rc;
-        a.a_glob = (T *)p;
+        a.a_glob = (T *)pa;
+        a.pos_out = (uint16_t *)pp;
     }
     const int grid = (int)Pb;
-    hipError_t e = pl.alds ? launch_rank_alds<T, true>(a, pl.maxch, pl.lds, grid, st)
-                           : launch_rank_alds<T, false>(a, pl.maxch, pl.lds, grid, st);
+    a.es_waves = kRankWaves;
+    a.off_ids = pl.rank.off_ids; a.off_cnt = pl.rank.off_cnt; a.off_misc = pl.rank.off_misc;
+    hipError_t e = pl.rank.alds
+        ? launch_by_maxch<T, true, true>(a, pl.maxch, pl.rank.lds, grid, st)
+        : launch_by_maxch<T, true, false>(a, pl.maxch, pl.rank.lds, grid, st);
     HIP_TRY(e);
+    if (G > 0) {
+        a.es_waves = pl.es.waves;
+        a.off_ids = pl.es.off_ids; a.off_cnt = pl.es.off_cnt; a.off_misc = pl.es.off_misc;
+        e = pl.es.alds ? launch_by_maxch<T, false, true>(a, pl.maxch, pl.es.lds, grid, st)
+                       : launch_by_maxch<T, false, false>(a, pl.maxch, pl.es.lds, grid, st);
+        HIP_TRY(e);
+    }
     return GSEA_OK;
 }
 
@@ -429,7 +547,10 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
         const size_t lds = align16(sizeof(T) * n) + 2 * sizeof(uint32_t) * (n + 1);
         const int64_t total = (P - 1) * G;
         int blocks = (int)std::min<int64_t>((total + 255) / 256, 4LL * d->cu_count);
-        if (lds <= 65536) {
+        if (lds <= (size_t)d->lds_max) {
+            HIP_TRY(hipFuncSetAttribute((const void *)k_fdr_hist<T, true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
             hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), lds, st,
                                es_pg, P, g, factor, (const T *)pthr, n, hub, hlb, signs);
         } else {
@@ -496,6 +617,9 @@ static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(plab, L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
 
+    SetTiers tiers;
+    rc = build_tiers<T>(d_off, d_genes, G, N, d, st, &tiers);
+    if (rc) return rc;
     StageClock clk{d, st};
     int64_t batches = 0;
     for (int64_t base = 0; base < P; base += Pb, ++batches) {
@@ -513,7 +637,7 @@ static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
         if ((rc = clk.end())) return rc;
         if ((rc = clk.begin(2))) return rc;
         rc = launch_rank<T>((const T *)pr, N, pb, base, d_off, d_genes, G, flags,
-                            (T *)pes, d, st);
+                            (T *)pes, d, st, &tiers);
         if (rc) return rc;
         if ((rc = clk.end())) return rc;
     }
@@ -624,7 +748,10 @@ static int rank_es_stage(const T *d_r, int64_t N, int64_t P, const int32_t *d_of
     DeviceState *d = nullptr;
     rc = device_state(&d);
     if (rc) return rc;
-    rc = launch_rank<T>(d_r, N, P, 0, d_off, d_genes, G, flags, d_es, d, st);
+    SetTiers tiers;
+    rc = build_tiers<T>(d_off, d_genes, G, N, d, st, &tiers);
+    if (rc) return rc;
+    rc = launch_rank<T>(d_r, N, P, 0, d_off, d_genes, G, flags, d_es, d, st, &tiers);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     return GSEA_OK;
@@ -640,7 +767,7 @@ static int argsort_stage(const T *d_keys, int64_t N, int64_t P, int flags,
     rc = device_state(&d);
     if (rc) return rc;
     rc = launch_rank<T>(d_keys, N, P, 0, nullptr, nullptr, 0, flags, nullptr, d, st,
-                        d_index);
+                        nullptr, d_index);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     return GSEA_OK;
@@ -691,7 +818,8 @@ int gsea_device_name(int device, char *name, size_t name_len, size_t *total_byte
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (name && name_len) {
-        std::snprintf(name, name_len, "%s", prop.name);
+        // some runtimes leave the marketing name empty: fall back to the ISA
+        std::snprintf(name, name_len, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     }
     if (total_bytes) *total_bytes = prop.totalGlobalMem;
     return GSEA_OK;

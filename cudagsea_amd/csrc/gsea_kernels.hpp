@@ -2,8 +2,8 @@
 // path:
 //   k_class_lists  permutation p -> ordered sample lists of both classes
 //   k_metric       14 local ranking metrics, many permutations per X tile
-//   k_rank_es      per permutation: LDS radix argsort of all genes, then the
-//                  running-sum enrichment score of every gene set
+//   k_rank         per permutation: LDS radix argsort of all genes -> rank rows
+//   k_es           per permutation: running-sum enrichment score of every set
 //   k_set_stats / k_perm_extrema / k_fwer / k_fdr_*   final statistics
 //   k_transpose    [P][G] -> [G][P] for the reference's ES dump layout
 #pragma once
@@ -371,10 +371,38 @@ struct RankArgs {
     T *es;                   // [P][G]
     T *a_glob;               // [Pb][N] |metric| by rank when it is not in LDS
     uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
+    uint16_t *pos_out;       // [Pb][N] rank of each gene (k_rank -> k_es)
     int asc;
     int es_waves, nw;        // wavefronts that score sets, bitmap words
     size_t off_ids, off_cnt, off_misc;
+    // work items: n_large sets (> kSmallSet genes, one wavefront each), then
+    // chunks of 64 small sets (one lane each) grouped by size class
+    int n_items, n_large;
+    const int32_t *large_ids;          // [n_large] set ids, largest first
+    const int32_t *small_ids;          // [n_chunks*64] set id or -1
+    const int32_t *small_k;            // [n_chunks*64] set sizes
+    const int32_t *small_gm;           // [32][small_stride] gene ids
+    int small_stride;
+    int chunk_end8, chunk_end16;       // chunk class boundaries
+    const T *decay;                    // [G] (T)(-1.0 / (double)(N - k_s))
 };
+
+constexpr int kSmallSet = 32;
+
+// gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
+// read row t of 64 consecutive slots
+__global__ void k_small_gm(const int32_t *__restrict__ off,
+                           const int32_t *__restrict__ genes,
+                           const int32_t *__restrict__ small_ids, int nslots,
+                           int32_t *__restrict__ gm) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots) return;
+    const int s = small_ids[slot];
+    const int o0 = s >= 0 ? off[s] : 0;
+    const int k = s >= 0 ? off[s + 1] - o0 : 0;
+    for (int t = 0; t < kSmallSet; ++t)
+        gm[(size_t)t * nslots + slot] = t < k ? genes[o0 + t] : 0;
+}
 
 template <class T>
 struct KeyOf;
@@ -383,14 +411,219 @@ struct KeyOf<float> { typedef uint32_t type; };
 template <>
 struct KeyOf<double> { typedef uint64_t type; };
 
+// ---- enrichment score of one set from its sorted hit ranks --------------
+// The walk of enrichment_scores.cuh:310-330 evaluated at the hits.  With
+// c = |r|/N_R formed in T as the reference does (N_R = the set's sum of |r|
+// rounded to T), Sa_j = sum of c over hits 0..j (rank order) and m = misses
+// before hit j (its rank minus j), the running sum just before hit j is
+// Sa_{j-1} + m*decay and just after it Sa_j + m*decay.  Sa and the
+// candidates are carried in double and each candidate is rounded to T before
+// the first-strictly-largest |value| comparison.
+template <class T>
+__device__ __forceinline__ void es_candidates(double &Sa, double c, int q, int j,
+                                              double decay, T &best) {
+    const int m = q - j;
+    const double md = m ? (double)m * decay : 0.0;   // decay = -inf when k = N
+    const T vb = (T)(Sa + md);
+    if (cuabs(vb) > cuabs(best)) best = vb;
+    Sa += c;
+    const T va = (T)(Sa + md);
+    if (cuabs(va) > cuabs(best)) best = va;
+}
+
+// in-register bitonic sort of NC keys (compile-time network)
+template <int NC>
+__device__ __forceinline__ void bitonic_sort(uint32_t (&v)[NC]) {
+#pragma unroll
+    for (int size = 2; size <= NC; size <<= 1) {
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint32_t x = v[i], y = v[j];
+                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+                    if ((i & size) == 0) { v[i] = lo; v[j] = hi; }
+                    else { v[i] = hi; v[j] = lo; }
+                }
+            }
+        }
+    }
+}
+
+// Sets of at most NC genes: one lane per set.  The lane gathers its hit
+// ranks, sorts them in registers and walks them in rank order.
+template <class T, int NC, class AF>
+__device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
+                                              const uint16_t *pos, AF &A,
+                                              int64_t gp, int chunk, int lane) {
+    const int slot = chunk * kWave + lane;
+    const int sid = a.small_ids[slot];
+    const int k = a.small_k[slot];
+    uint32_t q[NC];
+    const int32_t *gcol = a.small_gm + slot;
+    // opaque per call: keeps the compiler from hoisting 32 row offsets out of
+    // the work loop into (spilled) scalar registers
+    int stride = a.small_stride;
+    asm volatile("" : "+s"(stride));
+    // gather in groups of 8 so that only 8 loads (and addresses) are live
+#pragma unroll
+    for (int t0 = 0; t0 < NC; t0 += 8) {
+        int gg[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            gg[u] = t0 + u < k ? gcol[(size_t)(t0 + u) * stride] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            q[t0 + u] = t0 + u < k ? (uint32_t)pos[gg[u]] : 0xffffffffu;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    bitonic_sort<NC>(q);
+    double nr = 0.0;
+#pragma unroll
+    for (int t0 = 0; t0 < NC; t0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u < k) nr += (double)A((int)q[t0 + u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const T NRt = (T)nr;
+
+    const double decay = sid >= 0 ? (double)a.decay[sid] : 0.0;
+    T best = T(0);
+    double Sa = 0.0;
+#pragma unroll
+    for (int t0 = 0; t0 < NC; t0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u < k)
+                es_candidates<T>(Sa, (double)(A((int)q[t0 + u]) / NRt), (int)q[t0 + u],
+                                 t0 + u, decay, best);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (k == 0) best = (T)((double)a.N * decay);
+    if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
+}
+
+// Larger sets: one wavefront per set.  The k rank bits go into the
+// wavefront's N-bit bitmap; lane l owns words [l*W, l*W+W), so lane order is
+// rank order.  When the sorted hit list fits the bitmap's memory as 16-bit
+// ranks, lanes write it there compacted and then take balanced contiguous
+// blocks of it; otherwise every lane walks the hits of its own words.
+template <class T, bool ALDS, int W, class AF>
+__device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
+                                             const uint16_t *pos, AF &A,
+                                             uint32_t *bm, int64_t gp, int s,
+                                             int lane) {
+    const int N = a.N;
+    const int o0 = a.set_off[s];
+    const int k = a.set_off[s + 1] - o0;
+    double nr = 0.0;
+    for (int j0 = 0; j0 < k; j0 += 4 * kWave) {
+        int g[4], q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * kWave + lane;
+            g[u] = j < k ? a.set_genes[o0 + j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = g[u] >= 0 ? (int)pos[g[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (q[u] >= 0) {
+                atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31));
+                nr += (double)A(q[u]);
+            }
+        }
+    }
+    nr = wave_sum(nr);
+    const T NRt = (T)nr;
+
+    const double decay = (double)a.decay[s];
+    wave_sync_lds();
+    uint32_t wd[W];
+    int hits = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const int wi = lane * W + i;
+        wd[i] = wi < a.nw ? bm[wi] : 0u;
+        hits += __popc(wd[i]);
+    }
+    const int j0 = wave_excl_scan(hits);
+    T best = T(0);
+    if (k <= 2 * a.nw) {
+        // compact the sorted ranks into the bitmap's memory (all words are
+        // already in registers), then process balanced blocks
+        uint16_t *list = reinterpret_cast<uint16_t *>(bm);
+        wave_sync_lds();
+        int jw = j0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            uint32_t bits = wd[i];
+            while (bits) {
+                list[jw++] = (uint16_t)((lane * W + i) * 32 + __ffs(bits) - 1);
+                bits &= bits - 1u;
+            }
+        }
+        wave_sync_lds();
+        const int per = (k + kWave - 1) / kWave;
+        const int b0 = min(k, lane * per), b1 = min(k, b0 + per);
+        double sa = 0.0;
+        for (int j = b0; j < b1; ++j) sa += (double)(A((int)list[j]) / NRt);
+        double Sa = wave_excl_scan(sa);
+        for (int j = b0; j < b1; ++j) {
+            const int q = list[j];
+            es_candidates<T>(Sa, (double)(A(q) / NRt), q, j, decay, best);
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (lane * W + i < a.nw) bm[lane * W + i] = 0u;
+    } else {
+        // pass 1: this lane's sum of |r| in rank order
+        double sa = 0.0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            uint32_t bits = wd[i];
+            while (bits) {
+                sa += (double)(A((lane * W + i) * 32 + __ffs(bits) - 1) / NRt);
+                bits &= bits - 1u;
+            }
+        }
+        double Sa = wave_excl_scan(sa);
+        int jj = j0;
+        // pass 2: candidates at this lane's hits; clear the words
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            uint32_t bits = wd[i];
+            if (lane * W + i < a.nw && bits) bm[lane * W + i] = 0u;
+            while (bits) {
+                const int q = (lane * W + i) * 32 + __ffs(bits) - 1;
+                bits &= bits - 1u;
+                es_candidates<T>(Sa, (double)(A(q) / NRt), q, jj, decay, best);
+                ++jj;
+            }
+        }
+    }
+    // leftmost strict maximum of |best| across lanes (rank order)
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const T o = __shfl_down(best, off, kWave);
+        if (lane + off < kWave && cuabs(o) > cuabs(best)) best = o;
+    }
+    if (k == 0) best = (T)((double)N * decay);
+    if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
+    wave_sync_lds();
+}
+
 template <class T, bool ALDS, int MAXCH>
-__global__ __launch_bounds__(kRankThreads) void k_rank_es(RankArgs<T> a) {
+__global__ __launch_bounds__(kRankThreads) void k_rank(RankArgs<T> a) {
     typedef typename KeyOf<T>::type KeyT;
     constexpr int NPASS = (int)sizeof(KeyT);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // LDS: [keys | a_rank] (ALDS only) | ids -> pos | counters -> bitmaps | misc
     uint32_t *keys = reinterpret_cast<uint32_t *>(smem_raw);
-    float *a_lds = reinterpret_cast<float *>(smem_raw);
     uint16_t *ids = reinterpret_cast<uint16_t *>(smem_raw + a.off_ids);
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem_raw + a.off_cnt);
     int *misc = reinterpret_cast<int *>(smem_raw + a.off_misc);
@@ -508,99 +741,89 @@ __global__ __launch_bounds__(kRankThreads) void k_rank_es(RankArgs<T> a) {
             continue;
         }
 
-        // invert: pos[gene] = rank; |metric| laid out by rank (read from
-        // global memory: the LDS keys are overwritten by a_lds)
-        {
+        // |metric| by rank, then invert ids into pos[gene] = rank; both rows go
+        // to HBM for k_es
+        if (a.pos_out) {
+            T *arow = a.a_glob + (size_t)p * N;
             uint32_t gi[MAXCH];
 #pragma unroll
             for (int j = 0; j < MAXCH; ++j) {
                 const int i = tid + j * kRankThreads;
                 const uint32_t g = ids[i];
                 gi[j] = g;
-                const T av = g < (uint32_t)N ? (T)__builtin_fabs((double)rp[g]) : T(0);
-                if (ALDS) a_lds[i] = (float)av;
-                else if (i < N) a.a_glob[(size_t)p * N + i] = av;
+                if (i < N) {
+                    if (ALDS) arow[i] = (T)key_abs(keys[g], asc);
+                    else arow[i] = (T)__builtin_fabs((double)rp[g]);
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < MAXCH; ++j)
                 ids[gi[j]] = (uint16_t)(tid + j * kRankThreads);
             __syncthreads();
+            for (int g = tid; g < N; g += kRankThreads)
+                a.pos_out[(size_t)p * N + g] = ids[g];
         }
-        const uint16_t *pos = ids;
-        const T *a_rank = ALDS ? nullptr : a.a_glob + (size_t)p * N;
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
+// k_es: enrichment scores of every gene set for one permutation per
+// workgroup, from the rank rows k_rank wrote (pos[gene], |metric| by rank).
+// Both rows are staged in LDS (|metric| only in fp32; fp64 reads it from L2).
+// Wavefronts take work items from an LDS counter: large sets (one wavefront
+// each, bitmap scan) first, then chunks of 64 small sets (one lane each).
+// ===========================================================================
+template <class T, bool ALDS, int W>
+__global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *a_lds = reinterpret_cast<float *>(smem_raw);
+    uint16_t *pos = reinterpret_cast<uint16_t *>(smem_raw + a.off_ids);
+    uint32_t *bms = reinterpret_cast<uint32_t *>(smem_raw + a.off_cnt);
+    int *misc = reinterpret_cast<int *>(smem_raw + a.off_misc);
+    const int N = a.N;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+
+    for (int p = blockIdx.x; p < a.Pb; p += gridDim.x) {
+        const T *arow = a.a_glob + (size_t)p * N;
+        const uint16_t *prow = a.pos_out + (size_t)p * N;
+        for (int g = tid; g < N; g += kRankThreads) {
+            pos[g] = prow[g];
+            if (ALDS) a_lds[g] = (float)arow[g];
+        }
+        if (tid == 0) misc[0] = 0;
+        if (w < a.es_waves)
+            for (int i = lane; i < a.nw; i += kWave) bms[(size_t)w * a.nw + i] = 0u;
+        __syncthreads();
         auto A = [&](int q) -> T {
             if (ALDS) return (T)a_lds[q];
-            return a_rank[q];
+            return arow[q];
         };
-
         if (w < a.es_waves) {
-            uint32_t *bm = cnt + (size_t)w * a.nw;
-            for (int i = lane; i < a.nw; i += kWave) bm[i] = 0u;
-            wave_sync_lds();
-            const int W = (a.nw + kWave - 1) / kWave;
-            const int w0 = lane * W;
-            const int w1 = min(w0 + W, a.nw);
+            uint32_t *bm = bms + (size_t)w * a.nw;
             const int64_t gp = a.perm_base + p;
-            for (;;) {
-                int s = 0;
-                if (lane == 0) s = atomicAdd(&misc[0], 1);
-                s = __shfl(s, 0, kWave);
-                if (s >= a.G) break;
-                const int o0 = a.set_off[s];
-                const int k = a.set_off[s + 1] - o0;
-                double nr = 0.0;
-                for (int j = lane; j < k; j += kWave) {
-                    const int q = pos[a.set_genes[o0 + j]];
-                    atomicOr(&bm[q >> 5], 1u << (q & 31));
-                    nr += (double)A(q);
+            int item = 0;
+            if (lane == 0) item = atomicAdd(&misc[0], 1);
+            item = __shfl(item, 0, kWave);
+            while (item < a.n_items) {
+                // fetch the next work item now; its latency hides behind this one
+                int next = 0;
+                if (lane == 0) next = atomicAdd(&misc[0], 1);
+                if (item < a.n_large) {
+                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp, a.large_ids[item], lane);
+                } else {
+                    const int chunk = item - a.n_large;
+                    if (chunk < a.chunk_end8)
+                        es_small_sets<T, 8>(a, pos, A, gp, chunk, lane);
+                    else if (chunk < a.chunk_end16)
+                        es_small_sets<T, 16>(a, pos, A, gp, chunk, lane);
+                    else
+                        es_small_sets<T, 32>(a, pos, A, gp, chunk, lane);
                 }
-                nr = wave_sum(nr);
-                const T NRt = (T)nr;
-                const T decay = (T)(-1.0 / (double)(N - k));
-                wave_sync_lds();
-                int hits = 0;
-                double cs = 0.0;
-                for (int wi = w0; wi < w1; ++wi) {
-                    uint32_t bits = bm[wi];
-                    hits += __popc(bits);
-                    while (bits) {
-                        const int q = wi * 32 + __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        const T c = A(q) / NRt;
-                        cs += (double)c;
-                    }
-                }
-                const int j0 = wave_excl_scan(hits);
-                double Sd = wave_excl_scan(cs);
-                int jj = j0;
-                T best = T(0);
-                for (int wi = w0; wi < w1; ++wi) {
-                    uint32_t bits = bm[wi];
-                    bm[wi] = 0u;
-                    while (bits) {
-                        const int q = wi * 32 + __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        const int m = q - jj;
-                        const double md = m ? (double)m * (double)decay : 0.0;
-                        const T vb = (T)(Sd + md);
-                        if (cuabs(vb) > cuabs(best)) best = vb;
-                        const T c = A(q) / NRt;
-                        Sd += (double)c;
-                        const T va = (T)(Sd + md);
-                        if (cuabs(va) > cuabs(best)) best = va;
-                        ++jj;
-                    }
-                }
-                // leftmost strict maximum of |best| across lanes (rank order)
-#pragma unroll
-                for (int off = 1; off < kWave; off <<= 1) {
-                    const T o = __shfl_down(best, off, kWave);
-                    if (lane + off < kWave && cuabs(o) > cuabs(best)) best = o;
-                }
-                if (k == 0) best = (T)((double)N * (double)decay);
-                if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
-                wave_sync_lds();
+                item = __shfl(next, 0, kWave);
             }
         }
         __syncthreads();
@@ -736,8 +959,9 @@ __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
         const int s = (int)(e % G);
         const T z = es[p * G + s] * factor[s];
         if (z != z) continue;
-        const int ub = upper_bound_t(t, n, z);
         const int lb = lower_bound_t(t, n, z);
+        int ub = lb;                       // thresholds equal to z are rare
+        while (ub < n && !(z < t[ub])) ++ub;
         if (LDS) {
             atomicAdd(&h1[ub], 1u);
             atomicAdd(&h2[lb], 1u);
