@@ -160,27 +160,93 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// sum over the 64 lanes; every lane returns lane 0's result so that a
-// floating-point total is the same value on all lanes
-template <class V>
-__device__ __forceinline__ V wave_sum(V v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return __shfl(v, 0, kWave);
+// ---- cross-lane moves on the DPP path (VALU, no LDS traffic) -------------
+// dpp_move<CTRL, ROWS>(v): every lane of the rows selected by ROWS takes v
+// from the lane CTRL names; lanes with no such source, and lanes of the other
+// rows, get 0 (all-zero bits).  64-bit values move as two 32-bit halves.
+// KEEP: those lanes keep their own v instead (for idempotent combinations).
+template <int CTRL, int ROWS, bool KEEP>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(KEEP ? (int)v : 0, (int)v, CTRL, ROWS,
+                                                 0xf, false);
 }
 
-// exclusive prefix sum over lanes in lane order
+template <int CTRL, int ROWS, class V, bool KEEP = false>
+__device__ __forceinline__ V dpp_move(V v) {
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8, "32- or 64-bit lane values");
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(
+            V, dpp32<CTRL, ROWS, KEEP>(__builtin_bit_cast(uint32_t, v)));
+    } else {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = dpp32<CTRL, ROWS, KEEP>((uint32_t)b);
+        const uint64_t hi = dpp32<CTRL, ROWS, KEEP>((uint32_t)(b >> 32));
+        return __builtin_bit_cast(V, lo | (hi << 32));
+    }
+}
+
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppWaveShr1 = 0x138, kDppRowBcast15 = 0x142,
+              kDppRowBcast31 = 0x143;
+
+// value of lane l, as a wave-uniform value
+template <class V>
+__device__ __forceinline__ V read_lane(V v, int l) {
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8, "32- or 64-bit lane values");
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(
+            V, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    } else {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+        return __builtin_bit_cast(V, lo | (hi << 32));
+    }
+}
+
+// inclusive prefix combination in lane order, op(left, right) associative:
+// Hillis-Steele within each row of 16 lanes, then the row totals carried by
+// the row broadcasts.  Lanes without a left neighbour see a 0 operand, so
+// op(0, x) must be x for the values that occur.
+template <class V, class Op>
+__device__ __forceinline__ V wave_incl_scan_op(V x, Op op) {
+    x = op(dpp_move<kDppRowShr1, 0xf>(x), x);
+    x = op(dpp_move<kDppRowShr2, 0xf>(x), x);
+    x = op(dpp_move<kDppRowShr4, 0xf>(x), x);
+    x = op(dpp_move<kDppRowShr8, 0xf>(x), x);
+    x = op(dpp_move<kDppRowBcast15, 0xa>(x), x);
+    x = op(dpp_move<kDppRowBcast31, 0xc>(x), x);
+    return x;
+}
+
+template <class V>
+__device__ __forceinline__ V wave_incl_scan(V x) {
+    return wave_incl_scan_op(x, [](V l, V r) { return l + r; });
+}
+
+// sum over the 64 lanes in a fixed order, the same value on every lane
+template <class V>
+__device__ __forceinline__ V wave_sum(V v) {
+    return read_lane(wave_incl_scan(v), kWave - 1);
+}
+
+// op over all 64 lanes for an idempotent, commutative op (min, max),
+// returned on every lane
+template <class V, class Op>
+__device__ __forceinline__ V wave_reduce_idem(V x, Op op) {
+    x = op(dpp_move<kDppRowShr1, 0xf, V, true>(x), x);
+    x = op(dpp_move<kDppRowShr2, 0xf, V, true>(x), x);
+    x = op(dpp_move<kDppRowShr4, 0xf, V, true>(x), x);
+    x = op(dpp_move<kDppRowShr8, 0xf, V, true>(x), x);
+    x = op(dpp_move<kDppRowBcast15, 0xa, V, true>(x), x);
+    x = op(dpp_move<kDppRowBcast31, 0xc, V, true>(x), x);
+    return read_lane(x, kWave - 1);
+}
+
+// exclusive prefix sum over lanes in lane order (lane 0 gets 0)
 template <class V>
 __device__ __forceinline__ V wave_excl_scan(V v) {
-    const int lane = lane_id();
-    V inc = v;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        V o = __shfl_up(inc, off, kWave);
-        if (lane >= off) inc += o;
-    }
-    const V ex = __shfl_up(inc, 1, kWave);
-    return lane == 0 ? V(0) : ex;
+    return dpp_move<kDppWaveShr1, 0xf>(wave_incl_scan(v));
 }
 
 }  // namespace gsea

@@ -358,7 +358,8 @@ template <class T>
 static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     pl->maxch = pick_maxch(N);
     const size_t npad = (size_t)kRankThreads * pl->maxch;
-    pl->nw = (int)((npad + 31) / 32);
+    // bitmap words per wavefront: W = (maxch+1)/2 words per lane
+    pl->nw = kWave * ((pl->maxch + 1) / 2);
     const size_t counters = (size_t)kRankWaves * 256 * 4;
     // k_rank keeps fp32 keys in LDS; fp64 keys are recomputed from r
     pl->rank.alds = sizeof(T) == 4 && fit(npad * 4, npad, counters, d->lds_max, &pl->rank);

@@ -373,7 +373,7 @@ struct RankArgs {
     uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
     uint16_t *pos_out;       // [Pb][N] rank of each gene (k_rank -> k_es)
     int asc;
-    int es_waves, nw;        // wavefronts that score sets, bitmap words
+    int es_waves, nw;        // wavefronts that score sets, bitmap words (64*W)
     size_t off_ids, off_cnt, off_misc;
     // work items: n_large sets (> kSmallSet genes, one wavefront each), then
     // chunks of 64 small sets (one lane each) grouped by size class
@@ -388,6 +388,13 @@ struct RankArgs {
 };
 
 constexpr int kSmallSet = 32;
+// sets of kSmallSet+1 .. kSortedSet genes are sorted across one wavefront;
+// larger ones go through a rank bitmap
+#ifndef GSEA_SORTED_SET
+#define GSEA_SORTED_SET 256
+#endif
+constexpr int kSortedSet = GSEA_SORTED_SET;
+static_assert(kSortedSet % 64 == 0 && kSortedSet >= 128, "wave-sorted tier size");
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
 // read row t of 64 consecutive slots
@@ -418,17 +425,54 @@ struct KeyOf<double> { typedef uint64_t type; };
 // before hit j (its rank minus j), the running sum just before hit j is
 // Sa_{j-1} + m*decay and just after it Sa_j + m*decay.  Sa and the
 // candidates are carried in double and each candidate is rounded to T before
-// the first-strictly-largest |value| comparison.
+// the first-strictly-largest |value| comparison.  A slot with valid == false
+// leaves Sa and best unchanged, so unrolled loops need no branches.
 template <class T>
 __device__ __forceinline__ void es_candidates(double &Sa, double c, int q, int j,
-                                              double decay, T &best) {
+                                              double decay, T &best,
+                                              bool valid = true) {
     const int m = q - j;
     const double md = m ? (double)m * decay : 0.0;   // decay = -inf when k = N
     const T vb = (T)(Sa + md);
-    if (cuabs(vb) > cuabs(best)) best = vb;
-    Sa += c;
+    // |x| > |y| compares like the reference's abs(x) > abs(y), -0 and NaN
+    // included, and maps to one compare with |.| operand modifiers
+    if (valid && fabs(vb) > fabs(best)) best = vb;
+    Sa += valid ? c : 0.0;
     const T va = (T)(Sa + md);
-    if (cuabs(va) > cuabs(best)) best = va;
+    if (valid && fabs(va) > fabs(best)) best = va;
+}
+
+// c = |r| / N_R rounded to T, one set's N_R fixed at construction.  fp32
+// forms (double)a * fl64(1/N_R) and rounds it once to float: that product is
+// within 2^-52 (relative) of the exact quotient, and a quotient of two floats
+// is either a float or at least 2^-49 (relative) away from every float
+// rounding midpoint (a midpoint has a 25-bit odd significand; times the
+// divisor's significand it cannot fit the dividend's 24 bits), so the result
+// is the correctly rounded quotient the reference's division yields, special
+// values included (0/0, x/0, x/inf).  fp64 divides.
+template <class T>
+struct HitShare;
+template <>
+struct HitShare<float> {
+    double rn;
+    __device__ __forceinline__ explicit HitShare(float n) : rn(1.0 / (double)n) {}
+    __device__ __forceinline__ float operator()(float a) const {
+        return (float)((double)a * rn);
+    }
+};
+template <>
+struct HitShare<double> {
+    double n;
+    __device__ __forceinline__ explicit HitShare(double n_) : n(n_) {}
+    __device__ __forceinline__ double operator()(double a) const { return a / n; }
+};
+
+// leftmost lane value of largest |value| (first strict maximum in lane
+// order), returned on every lane.  Lane values here are never NaN or -0.
+template <class T>
+__device__ __forceinline__ T wave_first_absmax(T x) {
+    x = wave_incl_scan_op(x, [](T l, T r) { return fabs(r) > fabs(l) ? r : l; });
+    return read_lane(x, kWave - 1);
 }
 
 // in-register bitonic sort of NC keys (compile-time network)
@@ -453,7 +497,9 @@ __device__ __forceinline__ void bitonic_sort(uint32_t (&v)[NC]) {
 }
 
 // Sets of at most NC genes: one lane per set.  The lane gathers its hit
-// ranks, sorts them in registers and walks them in rank order.
+// ranks, sorts them in registers and walks them in rank order.  Rows t >= k
+// of the gathered gene ids hold gene 0, so every load is in bounds; those
+// slots get the sentinel rank (sorted last) and are masked in the sums.
 template <class T, int NC, class AF>
 __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
                                               const uint16_t *pos, AF &A,
@@ -472,147 +518,240 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     for (int t0 = 0; t0 < NC; t0 += 8) {
         int gg[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            gg[u] = t0 + u < k ? gcol[(size_t)(t0 + u) * stride] : 0;
+        for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            q[t0 + u] = t0 + u < k ? (uint32_t)pos[gg[u]] : 0xffffffffu;
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t r = pos[gg[u]];
+            q[t0 + u] = t0 + u < k ? r : 0xffffffffu;
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     bitonic_sort<NC>(q);
     double nr = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
+        T x[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (t0 + u < k) nr += (double)A((int)q[t0 + u]);
+        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < k ? (int)q[t0 + u] : 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) nr += t0 + u < k ? (double)x[u] : 0.0;
         __builtin_amdgcn_sched_barrier(0);
     }
-    const T NRt = (T)nr;
+    const HitShare<T> share((T)nr);
 
     const double decay = sid >= 0 ? (double)a.decay[sid] : 0.0;
     T best = T(0);
     double Sa = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
+        T x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < k ? (int)q[t0 + u] : 0);
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (t0 + u < k)
-                es_candidates<T>(Sa, (double)(A((int)q[t0 + u]) / NRt), (int)q[t0 + u],
-                                 t0 + u, decay, best);
+            es_candidates<T>(Sa, (double)share(x[u]), (int)q[t0 + u], t0 + u, decay,
+                             best, t0 + u < k);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (k == 0) best = (T)((double)a.N * decay);
     if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
 }
 
-// Larger sets: one wavefront per set.  The k rank bits go into the
-// wavefront's N-bit bitmap; lane l owns words [l*W, l*W+W), so lane order is
-// rank order.  When the sorted hit list fits the bitmap's memory as 16-bit
-// ranks, lanes write it there compacted and then take balanced contiguous
-// blocks of it; otherwise every lane walks the hits of its own words.
+// Sets of at most 64*E genes: one wavefront per set, E hit ranks per lane,
+// bitonic-sorted across the wavefront (blocked: after the sort lane l holds
+// sorted positions l*E .. l*E+E-1), so every lane walks E consecutive hits.
+template <int E>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[E], int lane) {
+#pragma unroll
+    for (int size = 2; size <= kWave * E; size <<= 1) {
+#pragma unroll
+        for (int stride = size / 2; stride > 0; stride >>= 1) {
+            if (stride >= E) {
+                const int ls = stride / E;
+                const bool lower = (lane & ls) == 0;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t o = (uint32_t)__shfl_xor((int)v[e], ls, kWave);
+                    const bool up = ((lane * E + e) & size) == 0;
+                    const uint32_t lo = v[e] < o ? v[e] : o, hi = v[e] < o ? o : v[e];
+                    v[e] = (lower == up) ? lo : hi;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int f = e ^ stride;
+                    if (f > e) {
+                        const bool up = ((lane * E + e) & size) == 0;
+                        const uint32_t x = v[e], y = v[f];
+                        const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+                        v[e] = up ? lo : hi;
+                        v[f] = up ? hi : lo;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class T, int E, class AF>
+__device__ __forceinline__ void es_wave_sorted(const RankArgs<T> &a,
+                                               const uint16_t *pos, AF &A,
+                                               int64_t gp, int s, int o0, int k,
+                                               int lane) {
+    uint32_t q[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int j = e * kWave + lane;
+        const uint32_t r = pos[a.set_genes[o0 + min(j, k - 1)]];
+        q[e] = j < k ? r : 0xffffffffu;
+    }
+    wave_bitonic<E>(q, lane);
+    T av[E];
+    double nr = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool v = lane * E + e < k;
+        const T x = A(v ? (int)q[e] : 0);
+        av[e] = v ? x : T(0);
+        nr += (double)av[e];
+    }
+    nr = wave_sum(nr);
+    const HitShare<T> share((T)nr);
+    const double decay = (double)a.decay[s];
+    double sa = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        av[e] = share(av[e]);
+        if (lane * E + e < k) sa += (double)av[e];
+    }
+    double Sa = wave_excl_scan(sa);
+    T best = T(0);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        es_candidates<T>(Sa, (double)av[e], (int)q[e], lane * E + e, decay, best,
+                         lane * E + e < k);
+    best = wave_first_absmax(best);
+    if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
+}
+
+// Phase A of a large set: B gene-id slots per lane (set entries j0 ..
+// j0 + 64*B - 1), the bits of their ranks set in the wavefront's bitmap and
+// their |r| summed.  Slots past the set's end repeat its last gene (genes of
+// a set are distinct; OR-ing its bit again changes nothing) and add 0.
+template <class T, int B, class AF>
+__device__ __forceinline__ void es_mark_hits(const int32_t *sg, const uint16_t *pos,
+                                             AF &A, uint32_t *bm, int k, int j0,
+                                             int lane, double &nr) {
+    int q[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) q[u] = sg[min(j0 + u * kWave + lane, k - 1)];
+#pragma unroll
+    for (int u = 0; u < B; ++u) q[u] = pos[q[u]];
+    T x[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        x[u] = A(q[u]);
+        atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31));
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+        nr += j0 + u * kWave + lane < k ? (double)x[u] : 0.0;
+}
+
+// One lane's walk over its hits in rank order.  The lane's W bitmap words
+// hold ranks [lane*W*32, (lane+1)*W*32); nz flags its nonempty words and j is
+// the set index of its first hit.  EXACT == false: L accumulates the lane's
+// sum of c from 0, vmin / vmax keep the smallest value just before a hit and
+// the largest just after one, of L + misses*decay (NaN means none yet:
+// fmin / fmax skip it, as the reference's comparison skips a NaN running
+// sum).  EXACT == true: L starts at the lane's Sa and every candidate goes
+// through es_candidates (first strict maximum, rounded to T).
+template <class T, int W, bool EXACT, class AF>
+__device__ __forceinline__ void es_walk(const uint32_t *bm, AF &A,
+                                        const HitShare<T> &share, int lane,
+                                        uint32_t nz, int hits, int j, double dec,
+                                        double &L, double &vmin, double &vmax,
+                                        T &best) {
+    uint32_t cur = 0;
+    int base = 0;
+    for (int t = 0; t < hits; ++t) {
+        if (cur == 0u) {
+            const int i = __ffs(nz) - 1;
+            nz &= nz - 1u;
+            cur = bm[lane * W + i];
+            base = (lane * W + i) * 32;
+        }
+        const int q = base + __ffs(cur) - 1;
+        cur &= cur - 1u;
+        const double c = (double)share(A(q));
+        if (EXACT) {
+            es_candidates<T>(L, c, q, j + t, dec, best);
+        } else {
+            const double md = (double)(q - j - t) * dec;
+            vmin = fmin(vmin, L + md);
+            L += c;
+            vmax = fmax(vmax, L + md);
+        }
+    }
+}
+
+// Larger sets: one wavefront per set.  Phase A sets the k rank bits in the
+// wavefront's N-bit bitmap (lane l owns words l*W .. l*W+W-1, so lanes own
+// consecutive rank ranges) and sums N_R.  Each lane then walks its own hits
+// once, keeping its sum of c and the extremes of its local running sum; one
+// wavefront scan of the sums places every lane's extremes on the set's
+// running sum.  Rounding to T is monotone, so the largest (smallest) rounded
+// candidate is the rounded largest (smallest) one, and the enrichment score
+// is whichever has the larger magnitude.  Only when both magnitudes are
+// equal does the order of occurrence decide (the reference keeps the first):
+// that case walks the hits again with the first-strict-maximum rule.
 template <class T, bool ALDS, int W, class AF>
 __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
-                                             int lane) {
-    const int N = a.N;
-    const int o0 = a.set_off[s];
-    const int k = a.set_off[s + 1] - o0;
+                                             int o0, int k, int lane) {
+    const int32_t *sg = a.set_genes + o0;
     double nr = 0.0;
-    for (int j0 = 0; j0 < k; j0 += 4 * kWave) {
-        int g[4], q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u * kWave + lane;
-            g[u] = j < k ? a.set_genes[o0 + j] : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = g[u] >= 0 ? (int)pos[g[u]] : -1;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (q[u] >= 0) {
-                atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31));
-                nr += (double)A(q[u]);
-            }
-        }
+    for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
+        const int ns = (k - j0 + kWave - 1) / kWave;   // slots with set entries
+        if (ns > 4) es_mark_hits<T, 8>(sg, pos, A, bm, k, j0, lane, nr);
+        else if (ns > 2) es_mark_hits<T, 4>(sg, pos, A, bm, k, j0, lane, nr);
+        else if (ns > 1) es_mark_hits<T, 2>(sg, pos, A, bm, k, j0, lane, nr);
+        else es_mark_hits<T, 1>(sg, pos, A, bm, k, j0, lane, nr);
     }
     nr = wave_sum(nr);
-    const T NRt = (T)nr;
+    const HitShare<T> share((T)nr);
+    // k == N: no misses; the reference's decay (-inf) is never applied
+    const double dec = k == a.N ? 0.0 : (double)a.decay[s];
 
-    const double decay = (double)a.decay[s];
     wave_sync_lds();
-    uint32_t wd[W];
     int hits = 0;
+    uint32_t nz = 0;
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-        const int wi = lane * W + i;
-        wd[i] = wi < a.nw ? bm[wi] : 0u;
-        hits += __popc(wd[i]);
+        const uint32_t w = bm[lane * W + i];
+        hits += __popc(w);
+        nz |= (w != 0u ? 1u : 0u) << i;
     }
-    const int j0 = wave_excl_scan(hits);
+    const int j = wave_excl_scan(hits);
+    double L = 0.0, vmin = __builtin_nan(""), vmax = __builtin_nan("");
     T best = T(0);
-    if (k <= 2 * a.nw) {
-        // compact the sorted ranks into the bitmap's memory (all words are
-        // already in registers), then process balanced blocks
-        uint16_t *list = reinterpret_cast<uint16_t *>(bm);
-        wave_sync_lds();
-        int jw = j0;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            uint32_t bits = wd[i];
-            while (bits) {
-                list[jw++] = (uint16_t)((lane * W + i) * 32 + __ffs(bits) - 1);
-                bits &= bits - 1u;
-            }
-        }
-        wave_sync_lds();
-        const int per = (k + kWave - 1) / kWave;
-        const int b0 = min(k, lane * per), b1 = min(k, b0 + per);
-        double sa = 0.0;
-        for (int j = b0; j < b1; ++j) sa += (double)(A((int)list[j]) / NRt);
-        double Sa = wave_excl_scan(sa);
-        for (int j = b0; j < b1; ++j) {
-            const int q = list[j];
-            es_candidates<T>(Sa, (double)(A(q) / NRt), q, j, decay, best);
-        }
-        wave_sync_lds();
-#pragma unroll
-        for (int i = 0; i < W; ++i)
-            if (lane * W + i < a.nw) bm[lane * W + i] = 0u;
+    es_walk<T, W, false>(bm, A, share, lane, nz, hits, j, dec, L, vmin, vmax, best);
+    const double Sa = wave_excl_scan(L);
+    const T hi = wave_reduce_idem((T)(Sa + vmax), [](T x, T y) { return fmax(x, y); });
+    const T lo = wave_reduce_idem((T)(Sa + vmin), [](T x, T y) { return fmin(x, y); });
+    const T ah = hi == hi ? fabs(hi) : T(0);
+    const T al = lo == lo ? fabs(lo) : T(0);
+    if (ah == al && ah != T(0)) {
+        double S2 = Sa;
+        es_walk<T, W, true>(bm, A, share, lane, nz, hits, j, dec, S2, vmin, vmax, best);
+        best = wave_first_absmax(best);
     } else {
-        // pass 1: this lane's sum of |r| in rank order
-        double sa = 0.0;
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            uint32_t bits = wd[i];
-            while (bits) {
-                sa += (double)(A((lane * W + i) * 32 + __ffs(bits) - 1) / NRt);
-                bits &= bits - 1u;
-            }
-        }
-        double Sa = wave_excl_scan(sa);
-        int jj = j0;
-        // pass 2: candidates at this lane's hits; clear the words
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-            uint32_t bits = wd[i];
-            if (lane * W + i < a.nw && bits) bm[lane * W + i] = 0u;
-            while (bits) {
-                const int q = (lane * W + i) * 32 + __ffs(bits) - 1;
-                bits &= bits - 1u;
-                es_candidates<T>(Sa, (double)(A(q) / NRt), q, jj, decay, best);
-                ++jj;
-            }
-        }
+        best = ah > al ? hi : (al > ah ? lo : T(0));
     }
-    // leftmost strict maximum of |best| across lanes (rank order)
+    wave_sync_lds();
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const T o = __shfl_down(best, off, kWave);
-        if (lane + off < kWave && cuabs(o) > cuabs(best)) best = o;
-    }
-    if (k == 0) best = (T)((double)N * decay);
+    for (int i = 0; i < W; ++i) bm[lane * W + i] = 0u;
     if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
     wave_sync_lds();
 }
@@ -813,7 +952,17 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                 int next = 0;
                 if (lane == 0) next = atomicAdd(&misc[0], 1);
                 if (item < a.n_large) {
-                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp, a.large_ids[item], lane);
+                    const int s = a.large_ids[item];
+                    const int o0 = a.set_off[s];
+                    const int k = a.set_off[s + 1] - o0;
+                    if (k > kSortedSet)
+                        es_large_set<T, ALDS, W>(a, pos, A, bm, gp, s, o0, k, lane);
+                    else if (k > 2 * kWave)
+                        es_wave_sorted<T, kSortedSet / kWave>(a, pos, A, gp, s, o0, k, lane);
+                    else if (k > kWave)
+                        es_wave_sorted<T, 2>(a, pos, A, gp, s, o0, k, lane);
+                    else
+                        es_wave_sorted<T, 1>(a, pos, A, gp, s, o0, k, lane);
                 } else {
                     const int chunk = item - a.n_large;
                     if (chunk < a.chunk_end8)
