@@ -382,9 +382,10 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
 // 64 sets of one size class (<= 8, <= 16, <= 32 genes).
 struct SetTiers {
     int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, stride = 0;
-    const int32_t *large_ids = nullptr, *small_ids = nullptr, *small_k = nullptr,
-                  *small_gm = nullptr;
-    const void *decay = nullptr;   // [G] of T
+    const int4 *large_desc = nullptr;
+    const int32_t *small_ids = nullptr, *small_k = nullptr, *small_gm = nullptr;
+    const void *decay = nullptr;         // [G] of T
+    const void *large_decay = nullptr;   // [n_large] of T
 };
 
 template <class T>
@@ -417,27 +418,39 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     t->chunk_end16 = ends[1];
     t->stride = (int)sid.size();
     const size_t nl = large.size(), ns = sid.size();
-    std::vector<int32_t> host(nl + 2 * ns);
-    std::copy(large.begin(), large.end(), host.begin());
-    std::copy(sid.begin(), sid.end(), host.begin() + nl);
-    std::copy(sk.begin(), sk.end(), host.begin() + nl + ns);
-    // per-set decay of the running sum, enrichment_scores.cuh:311
-    std::vector<T> decay((size_t)G);
+    // int32 words: large descriptors {set, offset, size, 0} | small ids | small sizes
+    std::vector<int32_t> host(4 * nl + 2 * ns);
+    for (size_t i = 0; i < nl; ++i) {
+        const int32_t s = large[i];
+        host[4 * i] = s;
+        host[4 * i + 1] = off[s];
+        host[4 * i + 2] = off[s + 1] - off[s];
+        host[4 * i + 3] = 0;
+    }
+    std::copy(sid.begin(), sid.end(), host.begin() + 4 * nl);
+    std::copy(sk.begin(), sk.end(), host.begin() + 4 * nl + ns);
+    // per-set decay of the running sum, enrichment_scores.cuh:311, by set id
+    // and again in large-item order
+    std::vector<T> decay((size_t)G + nl);
     for (int64_t s = 0; s < G; ++s)
         decay[(size_t)s] = (T)(-1.0 / (double)(N - (off[(size_t)s + 1] - off[(size_t)s])));
+    for (size_t i = 0; i < nl; ++i) decay[(size_t)G + i] = decay[(size_t)large[i]];
     void *pt, *pg, *pd;
     int rc = ensure(d, kBufTier, sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
-    if (!rc) rc = ensure(d, kBufDecay, sizeof(T) * std::max<int64_t>(G, 1), &pd);
+    if (!rc) rc = ensure(d, kBufDecay, sizeof(T) * std::max<size_t>(decay.size(), 1), &pd);
     if (!rc) rc = ensure(d, kBufTierGm, sizeof(int32_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
     if (rc) return rc;
     if (!host.empty())
         HIP_TRY(hipMemcpyAsync(pt, host.data(), sizeof(int32_t) * host.size(),
                                hipMemcpyHostToDevice, st));
-    if (G) HIP_TRY(hipMemcpyAsync(pd, decay.data(), sizeof(T) * G, hipMemcpyHostToDevice, st));
+    if (!decay.empty())
+        HIP_TRY(hipMemcpyAsync(pd, decay.data(), sizeof(T) * decay.size(),
+                               hipMemcpyHostToDevice, st));
     t->decay = pd;
-    t->large_ids = (const int32_t *)pt;
-    t->small_ids = (const int32_t *)pt + nl;
-    t->small_k = (const int32_t *)pt + nl + ns;
+    t->large_decay = (const T *)pd + G;
+    t->large_desc = (const int4 *)pt;
+    t->small_ids = (const int32_t *)pt + 4 * nl;
+    t->small_k = (const int32_t *)pt + 4 * nl + ns;
     t->small_gm = (const int32_t *)pg;
     if (ns) {
         hipLaunchKernelGGL(k_small_gm, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
@@ -467,7 +480,8 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
     a.idx_out = idx_out;
     a.n_large = tiers ? tiers->n_large : 0;
     a.n_items = tiers ? tiers->n_large + tiers->n_chunks : 0;
-    a.large_ids = tiers ? tiers->large_ids : nullptr;
+    a.large_desc = tiers ? tiers->large_desc : nullptr;
+    a.large_decay = tiers ? (const T *)tiers->large_decay : nullptr;
     a.small_ids = tiers ? tiers->small_ids : nullptr;
     a.small_k = tiers ? tiers->small_k : nullptr;
     a.small_gm = tiers ? tiers->small_gm : nullptr;

@@ -378,7 +378,9 @@ struct RankArgs {
     // work items: n_large sets (> kSmallSet genes, one wavefront each), then
     // chunks of 64 small sets (one lane each) grouped by size class
     int n_items, n_large;
-    const int32_t *large_ids;          // [n_large] set ids, largest first
+    const int4 *large_desc;            // [n_large] {set id, offset, size, 0},
+                                       // largest first
+    const T *large_decay;              // [n_large] decay of those sets
     const int32_t *small_ids;          // [n_chunks*64] set id or -1
     const int32_t *small_k;            // [n_chunks*64] set sizes
     const int32_t *small_gm;           // [32][small_stride] gene ids
@@ -388,13 +390,6 @@ struct RankArgs {
 };
 
 constexpr int kSmallSet = 32;
-// sets of kSmallSet+1 .. kSortedSet genes are sorted across one wavefront;
-// larger ones go through a rank bitmap
-#ifndef GSEA_SORTED_SET
-#define GSEA_SORTED_SET 256
-#endif
-constexpr int kSortedSet = GSEA_SORTED_SET;
-static_assert(kSortedSet % 64 == 0 && kSortedSet >= 128, "wave-sorted tier size");
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
 // read row t of 64 consecutive slots
@@ -557,83 +552,6 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
 }
 
-// Sets of at most 64*E genes: one wavefront per set, E hit ranks per lane,
-// bitonic-sorted across the wavefront (blocked: after the sort lane l holds
-// sorted positions l*E .. l*E+E-1), so every lane walks E consecutive hits.
-template <int E>
-__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[E], int lane) {
-#pragma unroll
-    for (int size = 2; size <= kWave * E; size <<= 1) {
-#pragma unroll
-        for (int stride = size / 2; stride > 0; stride >>= 1) {
-            if (stride >= E) {
-                const int ls = stride / E;
-                const bool lower = (lane & ls) == 0;
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t o = (uint32_t)__shfl_xor((int)v[e], ls, kWave);
-                    const bool up = ((lane * E + e) & size) == 0;
-                    const uint32_t lo = v[e] < o ? v[e] : o, hi = v[e] < o ? o : v[e];
-                    v[e] = (lower == up) ? lo : hi;
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const int f = e ^ stride;
-                    if (f > e) {
-                        const bool up = ((lane * E + e) & size) == 0;
-                        const uint32_t x = v[e], y = v[f];
-                        const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
-                        v[e] = up ? lo : hi;
-                        v[f] = up ? hi : lo;
-                    }
-                }
-            }
-        }
-    }
-}
-
-template <class T, int E, class AF>
-__device__ __forceinline__ void es_wave_sorted(const RankArgs<T> &a,
-                                               const uint16_t *pos, AF &A,
-                                               int64_t gp, int s, int o0, int k,
-                                               int lane) {
-    uint32_t q[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int j = e * kWave + lane;
-        const uint32_t r = pos[a.set_genes[o0 + min(j, k - 1)]];
-        q[e] = j < k ? r : 0xffffffffu;
-    }
-    wave_bitonic<E>(q, lane);
-    T av[E];
-    double nr = 0.0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const bool v = lane * E + e < k;
-        const T x = A(v ? (int)q[e] : 0);
-        av[e] = v ? x : T(0);
-        nr += (double)av[e];
-    }
-    nr = wave_sum(nr);
-    const HitShare<T> share((T)nr);
-    const double decay = (double)a.decay[s];
-    double sa = 0.0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        av[e] = share(av[e]);
-        if (lane * E + e < k) sa += (double)av[e];
-    }
-    double Sa = wave_excl_scan(sa);
-    T best = T(0);
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        es_candidates<T>(Sa, (double)av[e], (int)q[e], lane * E + e, decay, best,
-                         lane * E + e < k);
-    best = wave_first_absmax(best);
-    if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
-}
-
 // Phase A of a large set: B gene-id slots per lane (set entries j0 ..
 // j0 + 64*B - 1), the bits of their ranks set in the wavefront's bitmap and
 // their |r| summed.  Slots past the set's end repeat its last gene (genes of
@@ -658,16 +576,31 @@ __device__ __forceinline__ void es_mark_hits(const int32_t *sg, const uint16_t *
         nr += j0 + u * kWave + lane < k ? (double)x[u] : 0.0;
 }
 
+// All of a set's entries through es_mark_hits, 8 slots per lane per round;
+// the last round takes the smallest batch that covers the set's end.
+template <class T, class AF>
+__device__ __forceinline__ void es_mark_set(const int32_t *sg, const uint16_t *pos,
+                                            AF &A, uint32_t *bm, int k, int lane,
+                                            double &nr) {
+    for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
+        const int ns = (k - j0 + kWave - 1) / kWave;   // slots with set entries
+        if (ns > 4) es_mark_hits<T, 8>(sg, pos, A, bm, k, j0, lane, nr);
+        else if (ns > 2) es_mark_hits<T, 4>(sg, pos, A, bm, k, j0, lane, nr);
+        else if (ns > 1) es_mark_hits<T, 2>(sg, pos, A, bm, k, j0, lane, nr);
+        else es_mark_hits<T, 1>(sg, pos, A, bm, k, j0, lane, nr);
+    }
+}
+
 // One lane's walk over its hits in rank order.  The lane's W bitmap words
 // hold ranks [lane*W*32, (lane+1)*W*32); nz flags its nonempty words and j is
-// the set index of its first hit.  EXACT == false: L accumulates the lane's
+// the set index of its first hit.  Every word read is cleared.  EXACT == false: L accumulates the lane's
 // sum of c from 0, vmin / vmax keep the smallest value just before a hit and
 // the largest just after one, of L + misses*decay (NaN means none yet:
 // fmin / fmax skip it, as the reference's comparison skips a NaN running
 // sum).  EXACT == true: L starts at the lane's Sa and every candidate goes
 // through es_candidates (first strict maximum, rounded to T).
 template <class T, int W, bool EXACT, class AF>
-__device__ __forceinline__ void es_walk(const uint32_t *bm, AF &A,
+__device__ __forceinline__ void es_walk(uint32_t *bm, AF &A,
                                         const HitShare<T> &share, int lane,
                                         uint32_t nz, int hits, int j, double dec,
                                         double &L, double &vmin, double &vmax,
@@ -679,6 +612,7 @@ __device__ __forceinline__ void es_walk(const uint32_t *bm, AF &A,
             const int i = __ffs(nz) - 1;
             nz &= nz - 1u;
             cur = bm[lane * W + i];
+            bm[lane * W + i] = 0u;   // leaves the bitmap clear for the next set
             base = (lane * W + i) * 32;
         }
         const int q = base + __ffs(cur) - 1;
@@ -709,20 +643,14 @@ template <class T, bool ALDS, int W, class AF>
 __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
-                                             int o0, int k, int lane) {
+                                             int o0, int k, double decay, int lane) {
     const int32_t *sg = a.set_genes + o0;
     double nr = 0.0;
-    for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
-        const int ns = (k - j0 + kWave - 1) / kWave;   // slots with set entries
-        if (ns > 4) es_mark_hits<T, 8>(sg, pos, A, bm, k, j0, lane, nr);
-        else if (ns > 2) es_mark_hits<T, 4>(sg, pos, A, bm, k, j0, lane, nr);
-        else if (ns > 1) es_mark_hits<T, 2>(sg, pos, A, bm, k, j0, lane, nr);
-        else es_mark_hits<T, 1>(sg, pos, A, bm, k, j0, lane, nr);
-    }
+    es_mark_set<T>(sg, pos, A, bm, k, lane, nr);
     nr = wave_sum(nr);
     const HitShare<T> share((T)nr);
     // k == N: no misses; the reference's decay (-inf) is never applied
-    const double dec = k == a.N ? 0.0 : (double)a.decay[s];
+    const double dec = k == a.N ? 0.0 : decay;
 
     wave_sync_lds();
     int hits = 0;
@@ -743,15 +671,17 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
     const T ah = hi == hi ? fabs(hi) : T(0);
     const T al = lo == lo ? fabs(lo) : T(0);
     if (ah == al && ah != T(0)) {
+        // the walk cleared the bitmap: mark the hits again
+        wave_sync_lds();
+        double unused = 0.0;
+        es_mark_set<T>(sg, pos, A, bm, k, lane, unused);
+        wave_sync_lds();
         double S2 = Sa;
         es_walk<T, W, true>(bm, A, share, lane, nz, hits, j, dec, S2, vmin, vmax, best);
         best = wave_first_absmax(best);
     } else {
         best = ah > al ? hi : (al > ah ? lo : T(0));
     }
-    wave_sync_lds();
-#pragma unroll
-    for (int i = 0; i < W; ++i) bm[lane * W + i] = 0u;
     if (lane == 0) a.es[(size_t)gp * a.G + s] = best;
     wave_sync_lds();
 }
@@ -952,17 +882,9 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                 int next = 0;
                 if (lane == 0) next = atomicAdd(&misc[0], 1);
                 if (item < a.n_large) {
-                    const int s = a.large_ids[item];
-                    const int o0 = a.set_off[s];
-                    const int k = a.set_off[s + 1] - o0;
-                    if (k > kSortedSet)
-                        es_large_set<T, ALDS, W>(a, pos, A, bm, gp, s, o0, k, lane);
-                    else if (k > 2 * kWave)
-                        es_wave_sorted<T, kSortedSet / kWave>(a, pos, A, gp, s, o0, k, lane);
-                    else if (k > kWave)
-                        es_wave_sorted<T, 2>(a, pos, A, gp, s, o0, k, lane);
-                    else
-                        es_wave_sorted<T, 1>(a, pos, A, gp, s, o0, k, lane);
+                    const int4 d = a.large_desc[item];
+                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp, d.x, d.y, d.z,
+                                             (double)a.large_decay[item], lane);
                 } else {
                     const int chunk = item - a.n_large;
                     if (chunk < a.chunk_end8)
