@@ -31,6 +31,11 @@ MAX_GENES = 32768
 MAX_SAMPLES = 65535
 
 
+class gsea_extension(ctypes.Structure):
+    _fields_ = [("fdr", ctypes.c_double), ("z", ctypes.c_double),
+                ("nes_tol", ctypes.c_double)]
+
+
 class gsea_timings(ctypes.Structure):
     _fields_ = [("ms_lists", ctypes.c_double), ("ms_metric", ctypes.c_double),
                 ("ms_rank_es", ctypes.c_double), ("ms_stats", ctypes.c_double),
@@ -66,6 +71,10 @@ SIGNATURES = {
                                  _I, _P, _P, _P, _P]),
     "gsea_run_device_f64": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,
                                  _I, _P, _P, _P, _P]),
+    "gsea_run_budget_device_f32": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,
+                                        _I, _P, _P, _P, _P, _P, _P, _P]),
+    "gsea_run_budget_device_f64": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,
+                                        _I, _P, _P, _P, _P, _P, _P, _P]),
     "gsea_metric_f32": (_I, [_P, _I64, _I64, _P, _I64, _I64, _I, _I, _P, _P]),
     "gsea_metric_f64": (_I, [_P, _I64, _I64, _P, _I64, _I64, _I, _I, _P, _P]),
     "gsea_rank_es_f32": (_I, [_P, _I64, _I64, _P, _P, _I64, _I, _P, _P]),
@@ -76,7 +85,7 @@ SIGNATURES = {
                            ctypes.POINTER(_I64)]),
     "gsea_dump_es": (_I, [ctypes.c_char_p, _P, _I64, _I64, _I]),
     "gsea_mlp_num_params": (_I, []),
-    "gsea_mlp_forward": (_I, [_P, _P, _I64, _I, _P, _P]),
+    "gsea_mlp_forward": (_I, [_P, _P, _I64, _I, _P, _P, _P]),
     "gsea_mlp_grad": (_I, [_P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float), _P]),
     "gsea_mlp_adam": (_I, [_P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, _P]),
     "gsea_set_features": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P]),

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,14 +66,19 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kBufTier, kBufTierGm, kBufPos, kBufDecay, kNumBufs
+    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kNumBufs
 };
+
+// set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
+constexpr int kTierSlots = 24;
+enum TierBuf { kTierIds, kTierGm, kTierDecay, kNumTierBufs };
 
 struct DeviceState {
     bool init = false;
     int lds_max = 65536;
     int cu_count = 256;
     Buffer bufs[kNumBufs];
+    Buffer tier_bufs[kTierSlots][kNumTierBufs];
     std::vector<hipEvent_t> events;
 };
 
@@ -108,8 +114,7 @@ static int device_state(DeviceState **out) {
     return GSEA_OK;
 }
 
-static int ensure(DeviceState *d, BufId id, size_t bytes, void **out) {
-    Buffer &b = d->bufs[id];
+static int ensure_buf(Buffer &b, size_t bytes, void **out) {
     bytes = std::max<size_t>(bytes, 256);
     if (b.cap < bytes) {
         if (b.ptr) HIP_TRY(hipFree(b.ptr));
@@ -121,6 +126,10 @@ static int ensure(DeviceState *d, BufId id, size_t bytes, void **out) {
     }
     *out = b.ptr;
     return GSEA_OK;
+}
+
+static int ensure(DeviceState *d, BufId id, size_t bytes, void **out) {
+    return ensure_buf(d->bufs[id], bytes, out);
 }
 
 // events recorded around launches; durations summed per stage after sync
@@ -390,13 +399,22 @@ struct SetTiers {
 
 template <class T>
 static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
-                       int64_t N, DeviceState *d, hipStream_t st, SetTiers *t) {
+                       int64_t N, DeviceState *d, hipStream_t st, SetTiers *t,
+                       int slot = 0, const std::vector<int32_t> *subset = nullptr) {
+    if (slot < 0 || slot >= kTierSlots)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "too many set lists");
     std::vector<int32_t> off((size_t)G + 1);
     HIP_TRY(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (G + 1),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int32_t> all;
+    if (!subset) {
+        all.resize((size_t)G);
+        for (int64_t s = 0; s < G; ++s) all[(size_t)s] = (int32_t)s;
+        subset = &all;
+    }
     std::vector<int32_t> large, cls[3];
-    for (int64_t s = 0; s < G; ++s) {
+    for (int32_t s : *subset) {
         const int k = off[(size_t)s + 1] - off[(size_t)s];
         if (k < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
         if (k > kSmallSet) large.push_back((int32_t)s);
@@ -436,9 +454,10 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
         decay[(size_t)s] = (T)(-1.0 / (double)(N - (off[(size_t)s + 1] - off[(size_t)s])));
     for (size_t i = 0; i < nl; ++i) decay[(size_t)G + i] = decay[(size_t)large[i]];
     void *pt, *pg, *pd;
-    int rc = ensure(d, kBufTier, sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
-    if (!rc) rc = ensure(d, kBufDecay, sizeof(T) * std::max<size_t>(decay.size(), 1), &pd);
-    if (!rc) rc = ensure(d, kBufTierGm, sizeof(int32_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
+    Buffer *tb = d->tier_bufs[slot];
+    int rc = ensure_buf(tb[kTierIds], sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
+    if (!rc) rc = ensure_buf(tb[kTierDecay], sizeof(T) * std::max<size_t>(decay.size(), 1), &pd);
+    if (!rc) rc = ensure_buf(tb[kTierGm], sizeof(int32_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
     if (rc) return rc;
     if (!host.empty())
         HIP_TRY(hipMemcpyAsync(pt, host.data(), sizeof(int32_t) * host.size(),
@@ -463,21 +482,18 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
 }
 
 template <class T>
-static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
-                       const int32_t *d_off, const int32_t *d_genes, int64_t G,
-                       int flags, T *es_pg, DeviceState *d, hipStream_t st,
-                       const SetTiers *tiers, uint32_t *idx_out = nullptr) {
-    RankPlan pl;
-    int rc = plan_rank<T>(N, d, &pl);
-    if (rc) return rc;
-    RankArgs<T> a;
-    a.r = r; a.N = (int)N; a.Pb = (int)Pb; a.perm_base = perm_base;
+static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_t Pb,
+                           int64_t perm_base, const int32_t *d_off,
+                           const int32_t *d_genes, int64_t G, int flags, T *es_pg,
+                           const SetTiers *tiers) {
+    a.r = nullptr; a.N = (int)N; a.Pb = (int)Pb; a.perm_base = perm_base;
     a.G = (int)G; a.set_off = d_off; a.set_genes = d_genes; a.es = es_pg;
     a.asc = (flags & GSEA_FLAG_ASCENDING) ? 1 : 0;
     a.nw = pl.nw;
     a.a_glob = nullptr;
     a.pos_out = nullptr;
-    a.idx_out = idx_out;
+    a.p_lo = 0;
+    a.idx_out = nullptr;
     a.n_large = tiers ? tiers->n_large : 0;
     a.n_items = tiers ? tiers->n_large + tiers->n_chunks : 0;
     a.large_desc = tiers ? tiers->large_desc : nullptr;
@@ -489,6 +505,55 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
     a.chunk_end8 = tiers ? tiers->chunk_end8 : 0;
     a.chunk_end16 = tiers ? tiers->chunk_end16 : 0;
     a.decay = tiers ? (const T *)tiers->decay : nullptr;
+}
+
+// k_es over rank rows [row_lo, row_hi) of the current batch (permutations
+// perm_base + row) for the sets of `tiers`; the rows were written by k_rank
+// into the rank-row buffers, which hold rows_cap rows
+template <class T>
+static int launch_es_rows(int64_t N, int64_t rows_cap, int64_t perm_base,
+                          const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                          int flags, T *es_pg, DeviceState *d, hipStream_t st,
+                          const SetTiers *tiers, int64_t row_lo, int64_t row_hi) {
+    if (row_hi <= row_lo || !tiers || tiers->n_large + tiers->n_chunks == 0)
+        return GSEA_OK;
+    RankPlan pl;
+    int rc = plan_rank<T>(N, d, &pl);
+    if (rc) return rc;
+    RankArgs<T> a;
+    fill_rank_args<T>(a, pl, N, row_hi - row_lo, perm_base, d_off, d_genes, G, flags,
+                      es_pg, tiers);
+    void *pa = nullptr, *pp = nullptr;
+    rc = ensure(d, kBufAGlob, (size_t)rows_cap * N * sizeof(T), &pa);
+    if (!rc) rc = ensure(d, kBufPos, (size_t)rows_cap * N * sizeof(uint16_t), &pp);
+    if (rc) return rc;
+    a.a_glob = (T *)pa;
+    a.pos_out = (uint16_t *)pp;
+    a.p_lo = (int)row_lo;
+    a.es_waves = pl.es.waves;
+    a.off_ids = pl.es.off_ids; a.off_cnt = pl.es.off_cnt; a.off_misc = pl.es.off_misc;
+    const int grid = (int)(row_hi - row_lo);
+    hipError_t e = pl.es.alds
+        ? launch_by_maxch<T, false, true>(a, pl.maxch, pl.es.lds, grid, st)
+        : launch_by_maxch<T, false, false>(a, pl.maxch, pl.es.lds, grid, st);
+    HIP_TRY(e);
+    return GSEA_OK;
+}
+
+// k_rank over the Pb rows of a batch (rank rows kept for k_es), then k_es of
+// every row for the sets of `tiers` (none: ranks only)
+template <class T>
+static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
+                       const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                       int flags, T *es_pg, DeviceState *d, hipStream_t st,
+                       const SetTiers *tiers, uint32_t *idx_out = nullptr) {
+    RankPlan pl;
+    int rc = plan_rank<T>(N, d, &pl);
+    if (rc) return rc;
+    RankArgs<T> a;
+    fill_rank_args<T>(a, pl, N, Pb, perm_base, d_off, d_genes, G, flags, es_pg, tiers);
+    a.r = r;
+    a.idx_out = idx_out;
     if (G > 0) {
         // rank rows handed from k_rank to k_es: |metric| by rank, rank by gene
         void *pa = nullptr, *pp = nullptr;
@@ -505,13 +570,9 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
         ? launch_by_maxch<T, true, true>(a, pl.maxch, pl.rank.lds, grid, st)
         : launch_by_maxch<T, true, false>(a, pl.maxch, pl.rank.lds, grid, st);
     HIP_TRY(e);
-    if (G > 0) {
-        a.es_waves = pl.es.waves;
-        a.off_ids = pl.es.off_ids; a.off_cnt = pl.es.off_cnt; a.off_misc = pl.es.off_misc;
-        e = pl.es.alds ? launch_by_maxch<T, false, true>(a, pl.maxch, pl.es.lds, grid, st)
-                       : launch_by_maxch<T, false, false>(a, pl.maxch, pl.es.lds, grid, st);
-        HIP_TRY(e);
-    }
+    if (G > 0)
+        return launch_es_rows<T>(N, Pb, perm_base, d_off, d_genes, G, flags, es_pg, d, st,
+                                 tiers, 0, Pb);
     return GSEA_OK;
 }
 
@@ -520,7 +581,8 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
 // ---------------------------------------------------------------------------
 template <class T>
 static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
-                     DeviceState *d, hipStream_t st) {
+                     DeviceState *d, hipStream_t st,
+                     const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
     void *pf, *pmax, *pmin;
     int rc = ensure(d, kBufFactor, (size_t)G * sizeof(T), &pf);
     if (!rc) rc = ensure(d, kBufMax, (size_t)P * sizeof(T), &pmax);
@@ -529,13 +591,13 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
     T *factor = (T *)pf, *maxv = (T *)pmax, *minv = (T *)pmin;
     const int g = (int)G;
     hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 255) / 256), dim3(256), 0, st,
-                       es_pg, P, g, d_stats, factor);
+                       es_pg, P, g, d_stats, factor, d_budget, d_cv);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_perm_extrema<T>, dim3((unsigned)((P + 3) / 4)), dim3(256), 0,
-                       st, es_pg, P, g, factor, maxv, minv);
+                       st, es_pg, P, g, factor, d_budget, maxv, minv);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
-                       P, g, maxv, minv);
+                       P, g, maxv, minv, d_budget);
     HIP_TRY(hipGetLastError());
 
     // FDR: thresholds are the sorted non-NaN NES (sorted on the host: G values)
@@ -567,10 +629,12 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds));
             hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), lds, st,
-                               es_pg, P, g, factor, (const T *)pthr, n, hub, hlb, signs);
+                               es_pg, P, g, factor, d_budget, (const T *)pthr, n, hub, hlb,
+                               signs);
         } else {
             hipLaunchKernelGGL((k_fdr_hist<T, false>), dim3(blocks), dim3(256), 0, st,
-                               es_pg, P, g, factor, (const T *)pthr, n, hub, hlb, signs);
+                               es_pg, P, g, factor, d_budget, (const T *)pthr, n, hub, hlb,
+                               signs);
         }
         HIP_TRY(hipGetLastError());
     }
@@ -668,6 +732,213 @@ static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
         if ((rc = clk.end())) return rc;
     }
     HIP_TRY(hipStreamSynchronize(st));
+    return finish_timings(clk, t, batches);
+}
+
+// ---------------------------------------------------------------------------
+// budgeted run: set s is scored for permutations 0..budget[s]-1 only
+// ---------------------------------------------------------------------------
+
+// Sets (budget < P) to score for all permutations.  se = cv*|NES|/sqrt(B) is
+// a set's NES standard error.  Call rule (ext.fdr > 0): among the sets of
+// the same NES sign whose NES lies within z*se of this set's, FDR values on
+// both sides of ext.fdr (or equal to it) mean the set's call is not settled.
+// Precision rule (ext.nes_tol > 0): z*se > nes_tol.
+template <class T>
+static std::vector<int32_t> extension_sets(const T *st5, const float *cv,
+                                           const int32_t *B, int64_t G, int64_t P,
+                                           const gsea_extension &ext) {
+    const T *nes = st5 + G, *fdr = st5 + 4 * G;
+    std::vector<int32_t> pos, neg;
+    for (int64_t s = 0; s < G; ++s) {
+        if (!(nes[s] == nes[s]) || !(fdr[s] == fdr[s])) continue;
+        (nes[s] > T(0) ? pos : neg).push_back((int32_t)s);
+    }
+    auto by_nes = [&](int32_t a, int32_t b) { return nes[a] < nes[b]; };
+    std::sort(pos.begin(), pos.end(), by_nes);
+    std::sort(neg.begin(), neg.end(), by_nes);
+    std::vector<int32_t> out;
+    for (int64_t s = 0; s < G; ++s) {
+        if (B[s] >= P) continue;
+        const double n = (double)nes[s];
+        const double se = (double)cv[s] * std::fabs(n) / std::sqrt((double)B[s]);
+        bool grow = false;
+        if (ext.nes_tol > 0.0) grow = !(ext.z * se <= ext.nes_tol);
+        if (!grow && ext.fdr > 0.0 && n == n) {
+            const std::vector<int32_t> &v = n > 0.0 ? pos : neg;
+            const double lo = n - ext.z * se, hi = n + ext.z * se;
+            auto a = std::lower_bound(v.begin(), v.end(), lo, [&](int32_t x, double y) {
+                return (double)nes[x] < y;
+            });
+            bool below = false, above = false;
+            for (auto it = a; it != v.end() && (double)nes[*it] <= hi; ++it) {
+                const double f = (double)fdr[*it];
+                below = below || f <= ext.fdr;
+                above = above || f >= ext.fdr;
+            }
+            const double f0 = (double)fdr[s];
+            below = below || f0 <= ext.fdr;
+            above = above || f0 >= ext.fdr;
+            grow = below && above;
+        }
+        if (grow) out.push_back((int32_t)s);
+    }
+    return out;
+}
+
+template <class T>
+static int run_device_budget(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                             const int32_t *d_off, const int32_t *d_genes, int64_t G,
+                             int64_t P, int metric, int flags, const int32_t *budget,
+                             const gsea_extension *ext, int32_t *budget_out,
+                             T *d_es_pg, T *d_stats, hipStream_t st, gsea_timings *t) {
+    int rc = check_sizes(N, S, G, P, metric);
+    if (rc) return rc;
+    if (!dX || !labels || !d_off || !d_genes || !d_stats || !budget)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (G < 1 || P < 2) return fail(GSEA_ERR_INVALID_ARGUMENT, "budgeted run needs sets and P >= 2");
+    std::vector<int32_t> B(budget, budget + G);
+    for (int64_t s = 0; s < G; ++s)
+        if (B[(size_t)s] < 2 || B[(size_t)s] > P || P % B[(size_t)s] != 0)
+            return fail(GSEA_ERR_INVALID_ARGUMENT,
+                        "each budget must divide num_perms and be at least 2");
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    Labels L;
+    rc = prepare_labels(labels, S, flags, &L);
+    if (rc) return rc;
+
+    const int64_t Pb = batch_perms(N, P, sizeof(T));
+    void *plab, *psig, *plists, *pr, *pes, *pbud, *pcv;
+    rc = ensure(d, kBufLabels, (size_t)S, &plab);
+    if (!rc) rc = ensure(d, kBufSig, (size_t)Pb * S, &psig);
+    if (!rc) rc = ensure(d, kBufLists, (size_t)Pb * L.lstride * 2, &plists);
+    if (!rc) rc = ensure(d, kBufR, (size_t)Pb * N * sizeof(T), &pr);
+    if (!rc) rc = ensure(d, kBufES, (size_t)P * G * sizeof(T), &pes);
+    if (!rc) rc = ensure(d, kBufBudget, sizeof(int32_t) * G, &pbud);
+    if (!rc) rc = ensure(d, kBufCv, sizeof(float) * G, &pcv);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(plab, L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
+    StageClock clk{d, st};
+
+    // one set list per distinct budget (largest first), tier slots 1..
+    auto make_levels = [&](const std::vector<int32_t> &ids, const std::vector<int32_t> &bud,
+                           int slot0, std::vector<int64_t> *lv,
+                           std::vector<SetTiers> *tiers) -> int {
+        std::vector<int64_t> vals;
+        for (int32_t s : ids) vals.push_back(bud[(size_t)s]);
+        std::sort(vals.begin(), vals.end(), std::greater<int64_t>());
+        vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+        if (slot0 + (int)vals.size() > kTierSlots)
+            return fail(GSEA_ERR_INVALID_ARGUMENT, "too many distinct budgets");
+        lv->assign(vals.begin(), vals.end());
+        tiers->assign(vals.size(), SetTiers());
+        for (size_t l = 0; l < vals.size(); ++l) {
+            std::vector<int32_t> sub;
+            for (int32_t s : ids)
+                if (bud[(size_t)s] == vals[l]) sub.push_back(s);
+            int r = build_tiers<T>(d_off, d_genes, G, N, d, st, &(*tiers)[l],
+                                   slot0 + (int)l, &sub);
+            if (r) return r;
+        }
+        return GSEA_OK;
+    };
+    std::vector<int32_t> ids((size_t)G);
+    for (int64_t s = 0; s < G; ++s) ids[(size_t)s] = (int32_t)s;
+    std::vector<int64_t> lv;
+    std::vector<SetTiers> tiers;
+    if ((rc = make_levels(ids, B, 1, &lv, &tiers))) return rc;
+
+    // metric and rank rows of batch [base, base + pb); k_es of the given
+    // levels over rows [lo_l - base, hi_l - base) of it
+    auto batch = [&](int64_t base, int64_t pb) -> int {
+        int r;
+        if ((r = clk.begin(0))) return r;
+        hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((pb + 63) / 64)), dim3(64), 0, st,
+                           (const uint8_t *)plab, (int)S, base, (int)pb, (uint8_t *)psig,
+                           L.offB, (uint16_t *)plists, L.lstride);
+        HIP_TRY(hipGetLastError());
+        if ((r = clk.end())) return r;
+        if ((r = clk.begin(1))) return r;
+        r = launch_metric<T>(dX, N, S, L, (const uint16_t *)plists, pb, metric, (T *)pr, d, st);
+        if (r) return r;
+        if ((r = clk.end())) return r;
+        if ((r = clk.begin(2))) return r;
+        r = launch_rank<T>((const T *)pr, N, pb, base, d_off, d_genes, G, flags, (T *)pes,
+                           d, st, nullptr);
+        if (r) return r;
+        return clk.end();
+    };
+    auto score = [&](int64_t base, int64_t pb, const std::vector<int64_t> &lo,
+                     const std::vector<int64_t> &hi, std::vector<SetTiers> &tv) -> int {
+        int r;
+        if ((r = clk.begin(2))) return r;
+        for (size_t l = 0; l < tv.size(); ++l) {
+            const int64_t a = std::max(lo[l], base) - base;
+            const int64_t b = std::min(hi[l], base + pb) - base;
+            r = launch_es_rows<T>(N, Pb, base, d_off, d_genes, G, flags, (T *)pes, d, st,
+                                  &tv[l], a, b);
+            if (r) return r;
+        }
+        return clk.end();
+    };
+
+    int64_t batches = 0, resident = -1;
+    const std::vector<int64_t> zeros(lv.size(), 0);
+    for (int64_t base = 0; base < lv[0]; base += Pb, ++batches) {
+        const int64_t pb = std::min(Pb, P - base);
+        if ((rc = batch(base, pb))) return rc;
+        if ((rc = score(base, pb, zeros, lv, tiers))) return rc;
+        resident = base;
+    }
+    HIP_TRY(hipMemcpyAsync(pbud, B.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
+    if ((rc = clk.begin(3))) return rc;
+    rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, (const int32_t *)pbud,
+                      (float *)pcv);
+    if (rc) return rc;
+    if ((rc = clk.end())) return rc;
+
+    // extension: sets whose significance call could change within the
+    // uncertainty of their NES, or whose NES is imprecise, get all P
+    // permutations
+    if (ext) {
+        std::vector<T> st5((size_t)5 * G);
+        std::vector<float> cv((size_t)G);
+        HIP_TRY(hipMemcpyAsync(st5.data(), d_stats, sizeof(T) * 5 * G, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(cv.data(), pcv, sizeof(float) * G, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const std::vector<int32_t> grow =
+            extension_sets(st5.data(), cv.data(), B.data(), G, P, *ext);
+        std::vector<int32_t> ext_ids, oldB = B;
+        for (int32_t s : grow) { ext_ids.push_back(s); B[(size_t)s] = (int32_t)P; }
+        if (!ext_ids.empty()) {
+            // groups by previous budget: score permutations [old, P)
+            std::vector<int64_t> elv;
+            std::vector<SetTiers> etiers;
+            if ((rc = make_levels(ext_ids, oldB, 1 + (int)lv.size(), &elv, &etiers))) return rc;
+            const std::vector<int64_t> ehi(elv.size(), P);
+            const int64_t first = elv.back();   // smallest previous budget
+            for (int64_t base = (first / Pb) * Pb; base < P; base += Pb) {
+                const int64_t pb = std::min(Pb, P - base);
+                if (base != resident) {
+                    if ((rc = batch(base, pb))) return rc;
+                    resident = base;
+                }
+                if ((rc = score(base, pb, elv, ehi, etiers))) return rc;
+            }
+            HIP_TRY(hipMemcpyAsync(pbud, B.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
+            if ((rc = clk.begin(3))) return rc;
+            rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, (const int32_t *)pbud,
+                              (float *)pcv);
+            if (rc) return rc;
+            if ((rc = clk.end())) return rc;
+        }
+    }
+    if (d_es_pg)
+        HIP_TRY(hipMemcpyAsync(d_es_pg, pes, sizeof(T) * P * G, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (budget_out) std::copy(B.begin(), B.end(), budget_out);
     return finish_timings(clk, t, batches);
 }
 
@@ -874,6 +1145,32 @@ int gsea_run_device_f64(const double *dX, int64_t N, int64_t S, const uint8_t *l
     std::lock_guard<std::mutex> lk(g_mu);
     return run_device<double>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
                               d_es_out, d_stats, (hipStream_t)stream, t);
+}
+
+int gsea_run_budget_device_f32(const float *dX, int64_t N, int64_t S,
+                               const uint8_t *labels, const int32_t *d_off,
+                               const int32_t *d_genes, int64_t G, int64_t P, int metric,
+                               int flags, const int32_t *budget,
+                               const gsea_extension *ext, int32_t *budget_out,
+                               float *d_es_pg, float *d_stats, void *stream,
+                               gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_device_budget<float>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
+                                    budget, ext, budget_out, d_es_pg, d_stats,
+                                    (hipStream_t)stream, t);
+}
+
+int gsea_run_budget_device_f64(const double *dX, int64_t N, int64_t S,
+                               const uint8_t *labels, const int32_t *d_off,
+                               const int32_t *d_genes, int64_t G, int64_t P, int metric,
+                               int flags, const int32_t *budget,
+                               const gsea_extension *ext, int32_t *budget_out,
+                               double *d_es_pg, double *d_stats, void *stream,
+                               gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_device_budget<double>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
+                                     budget, ext, budget_out, d_es_pg, d_stats,
+                                     (hipStream_t)stream, t);
 }
 
 int gsea_metric_f32(const float *dX, int64_t N, int64_t S, const uint8_t *labels,

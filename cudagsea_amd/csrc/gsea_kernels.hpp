@@ -372,6 +372,7 @@ struct RankArgs {
     T *a_glob;               // [Pb][N] |metric| by rank when it is not in LDS
     uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
     uint16_t *pos_out;       // [Pb][N] rank of each gene (k_rank -> k_es)
+    int p_lo;                // k_es: first rank row (and permutation) it scores
     int asc;
     int es_waves, nw;        // wavefronts that score sets, bitmap words (64*W)
     size_t off_ids, off_cnt, off_misc;
@@ -857,8 +858,8 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
     const int w = tid / kWave;
 
     for (int p = blockIdx.x; p < a.Pb; p += gridDim.x) {
-        const T *arow = a.a_glob + (size_t)p * N;
-        const uint16_t *prow = a.pos_out + (size_t)p * N;
+        const T *arow = a.a_glob + (size_t)(a.p_lo + p) * N;
+        const uint16_t *prow = a.pos_out + (size_t)(a.p_lo + p) * N;
         for (int g = tid; g < N; g += kRankThreads) {
             pos[g] = prow[g];
             if (ALDS) a_lds[g] = (float)arow[g];
@@ -873,7 +874,7 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
         };
         if (w < a.es_waves) {
             uint32_t *bm = bms + (size_t)w * a.nw;
-            const int64_t gp = a.perm_base + p;
+            const int64_t gp = a.perm_base + a.p_lo + p;
             int item = 0;
             if (lane == 0) item = atomicAdd(&misc[0], 1);
             item = __shfl(item, 0, kWave);
@@ -907,47 +908,66 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
 // ===========================================================================
 
 // per set: ES, NES (naive sequential sum of |ES| in permutation order,
-// statistics.cuh:60-70), NP (statistics.cuh:73-77)
+// statistics.cuh:60-70), NP (statistics.cuh:73-77).  budget (optional):
+// set s was scored for permutations 0..budget[s]-1 only and its statistics
+// use those; cv (optional, with budget) receives std/mean of |null ES|.
 template <class T>
 __global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
-                            T *__restrict__ out, T *__restrict__ factor) {
+                            T *__restrict__ out, T *__restrict__ factor,
+                            const int32_t *__restrict__ budget,
+                            float *__restrict__ cv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G) return;
+    const int64_t B = budget ? (int64_t)budget[s] : P;
     const T e0 = es[s];
     const T ref = __builtin_fabs(e0);
     T sum = T(0);
+    double s1 = 0.0, s2 = 0.0;
     int64_t c = 0;
     int64_t p = 0;
-    for (; p + 4 <= P; p += 4) {
+    for (; p + 4 <= B; p += 4) {
         const T v0 = __builtin_fabs(es[(p + 0) * G + s]);
         const T v1 = __builtin_fabs(es[(p + 1) * G + s]);
         const T v2 = __builtin_fabs(es[(p + 2) * G + s]);
         const T v3 = __builtin_fabs(es[(p + 3) * G + s]);
         sum += v0; sum += v1; sum += v2; sum += v3;
         c += (v0 >= ref) + (v1 >= ref) + (v2 >= ref) + (v3 >= ref);
+        if (cv) {
+            s1 += (double)v0 + (double)v1 + (double)v2 + (double)v3;
+            s2 += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
+        }
     }
-    for (; p < P; ++p) {
+    for (; p < B; ++p) {
         const T v = __builtin_fabs(es[p * G + s]);
         sum += v;
         c += v >= ref;
+        if (cv) { s1 += (double)v; s2 += (double)v * v; }
     }
-    const T f = (T)P / sum;
+    const T f = (T)B / sum;
     out[s] = e0;
     out[G + s] = e0 * f;
-    out[2 * G + s] = (T)c / (T)P;
+    out[2 * G + s] = (T)c / (T)B;
     factor[s] = f;
+    if (cv) {
+        const double m = s1 / (double)B;
+        const double var = s2 / (double)B - m * m;
+        cv[s] = (float)(m > 0.0 && var > 0.0 ? sqrt(var) / m : 0.0);
+    }
 }
 
-// per permutation: extrema of normalised ES over sets (statistics.cuh:85-97)
+// per permutation: extrema of normalised ES over sets (statistics.cuh:85-97),
+// over the sets scored for that permutation when budgets are given
 template <class T>
 __global__ void k_perm_extrema(const T *__restrict__ es, int64_t P, int G,
                                const T *__restrict__ factor,
+                               const int32_t *__restrict__ budget,
                                T *__restrict__ maxv, T *__restrict__ minv) {
     const int64_t p = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (p >= P) return;
     const int lane = lane_id();
     T mx = -__builtin_inf(), mn = __builtin_inf();
     for (int s = lane; s < G; s += kWave) {
+        if (budget && p >= budget[s]) continue;
         const T v = es[p * G + s] * factor[s];
         mx = mx < v ? v : mx;
         mn = mn > v ? v : mn;
@@ -962,20 +982,23 @@ __global__ void k_perm_extrema(const T *__restrict__ es, int64_t P, int G,
     if (lane == 0) { maxv[p] = mx; minv[p] = mn; }
 }
 
-// FWER, statistics.cuh:100-110
+// FWER, statistics.cuh:100-110 (over the set's own permutations when
+// budgets are given)
 template <class T>
 __global__ void k_fwer(T *__restrict__ out, int64_t P, int G,
-                       const T *__restrict__ maxv, const T *__restrict__ minv) {
+                       const T *__restrict__ maxv, const T *__restrict__ minv,
+                       const int32_t *__restrict__ budget) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G) return;
+    const int64_t B = budget ? (int64_t)budget[s] : P;
     const T score = out[G + s];
     int64_t c = 0;
     if (score > T(0)) {
-        for (int64_t p = 0; p < P; ++p) c += maxv[p] >= score;
+        for (int64_t p = 0; p < B; ++p) c += maxv[p] >= score;
     } else {
-        for (int64_t p = 0; p < P; ++p) c += minv[p] <= score;
+        for (int64_t p = 0; p < B; ++p) c += minv[p] <= score;
     }
-    out[3 * G + s] = (T)c / (T)P;
+    out[3 * G + s] = (T)c / (T)B;
 }
 
 template <class T>
@@ -1006,6 +1029,7 @@ __device__ __forceinline__ int upper_bound_t(const T *v, int n, T x) {
 template <class T, bool LDS>
 __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
                            const T *__restrict__ factor,
+                           const int32_t *__restrict__ budget,
                            const T *__restrict__ thr, int n,
                            unsigned long long *__restrict__ hub,
                            unsigned long long *__restrict__ hlb,
@@ -1028,20 +1052,27 @@ __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = 1 + e / G;
         const int s = (int)(e % G);
+        // a set scored for B of the P permutations stands for P/B null
+        // samples per score (budgets divide P)
+        uint32_t w = 1;
+        if (budget) {
+            if (p >= budget[s]) continue;
+            w = (uint32_t)(P / budget[s]);
+        }
         const T z = es[p * G + s] * factor[s];
         if (z != z) continue;
         const int lb = lower_bound_t(t, n, z);
         int ub = lb;                       // thresholds equal to z are rare
         while (ub < n && !(z < t[ub])) ++ub;
         if (LDS) {
-            atomicAdd(&h1[ub], 1u);
-            atomicAdd(&h2[lb], 1u);
+            atomicAdd(&h1[ub], w);
+            atomicAdd(&h2[lb], w);
         } else {
-            atomicAdd(&hub[ub], 1ull);
-            atomicAdd(&hlb[lb], 1ull);
+            atomicAdd(&hub[ub], (unsigned long long)w);
+            atomicAdd(&hlb[lb], (unsigned long long)w);
         }
-        npos += z >= T(0);
-        nneg += z <= T(0);
+        npos += z >= T(0) ? w : 0u;
+        nneg += z <= T(0) ? w : 0u;
     }
     npos = wave_sum(npos);
     nneg = wave_sum(nneg);

@@ -108,6 +108,13 @@ __global__ void k_set_features(const float *__restrict__ r0,
     f[15] = (float)fmax(fmin((double)glob[3], 10.0), -10.0);
 }
 
+// rank of each gene from the gene at each rank
+__global__ void k_invert_ranks(const int32_t *__restrict__ rank_genes, int N,
+                               int32_t *__restrict__ pos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) pos[rank_genes[i]] = i;
+}
+
 // global summaries of the observed metric (one workgroup)
 __global__ void k_metric_summary(const float *__restrict__ r0, int N,
                                  float *__restrict__ glob) {
@@ -179,11 +186,15 @@ struct MlpScratch {
     float *z1, *h1, *z2, *h2, *d1, *d2, *d3;   // [B][kH] except d3 [B][kOut]
 };
 
+// norm (optional, inference): x_mean[16] x_inv_std[16] y_scale[4] y_shift[4]
+constexpr int kNormLen = 2 * kIn + 2 * kOut;
+
 template <bool BACKWARD>
 __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
         const float *__restrict__ params, const float *__restrict__ x,
         const float *__restrict__ t, int B, float *__restrict__ y,
-        MlpScratch sc, float *__restrict__ loss_part) {
+        MlpScratch sc, float *__restrict__ loss_part,
+        const float *__restrict__ norm) {
     __shared__ float W[kNumParams];
     __shared__ float red[256 / kWave];
     for (int i = threadIdx.x; i < kNumParams; i += blockDim.x) W[i] = params[i];
@@ -193,7 +204,10 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
     if (b < B) {
         float xi[kIn];
 #pragma unroll
-        for (int j = 0; j < kIn; ++j) xi[j] = x[(size_t)b * kIn + j];
+        for (int j = 0; j < kIn; ++j) {
+            xi[j] = x[(size_t)b * kIn + j];
+            if (norm) xi[j] = (xi[j] - norm[j]) * norm[kIn + j];
+        }
         float h1[kH];
         for (int i = 0; i < kH; ++i) {
             float z = W[oB1 + i];
@@ -223,7 +237,8 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
 #pragma unroll 8
             for (int j = 0; j < kH; ++j) z += W[oW3 + o * kH + j] * h2[j];
             out[o] = z;
-            if (y) y[(size_t)b * kOut + o] = z;
+            if (y) y[(size_t)b * kOut + o] =
+                norm ? z * norm[2 * kIn + o] + norm[2 * kIn + kOut + o] : z;
         }
         if (BACKWARD) {
             // mean squared error over B x kOut values
@@ -304,7 +319,7 @@ __global__ void k_adam(float *__restrict__ params, float *__restrict__ m,
 // products accumulated in fp32
 __global__ __launch_bounds__(256) void k_mlp_infer_bf16(
         const float *__restrict__ params, const float *__restrict__ x, int B,
-        float *__restrict__ y) {
+        float *__restrict__ y, const float *__restrict__ norm) {
     __shared__ __hip_bfloat16 W[kNumParams];
     for (int i = threadIdx.x; i < kNumParams; i += blockDim.x)
         W[i] = __float2bfloat16(params[i]);
@@ -313,7 +328,11 @@ __global__ __launch_bounds__(256) void k_mlp_infer_bf16(
     if (b >= B) return;
     float xi[kIn];
 #pragma unroll
-    for (int j = 0; j < kIn; ++j) xi[j] = __bfloat162float(__float2bfloat16(x[(size_t)b * kIn + j]));
+    for (int j = 0; j < kIn; ++j) {
+        float v = x[(size_t)b * kIn + j];
+        if (norm) v = (v - norm[j]) * norm[kIn + j];
+        xi[j] = __bfloat162float(__float2bfloat16(v));
+    }
     float h1[kH];
     for (int i = 0; i < kH; ++i) {
         float z = __bfloat162float(W[oB1 + i]);
@@ -333,7 +352,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer_bf16(
         float z = __bfloat162float(W[oB3 + o]);
 #pragma unroll 8
         for (int j = 0; j < kH; ++j) z += __bfloat162float(W[oW3 + o * kH + j]) * h2[j];
-        y[(size_t)b * kOut + o] = z;
+        y[(size_t)b * kOut + o] = norm ? z * norm[2 * kIn + o] + norm[2 * kIn + kOut + o] : z;
     }
 }
 
@@ -355,7 +374,7 @@ struct MlpWork {
     float *buf = nullptr;
     size_t cap = 0;   // floats
 };
-static MlpWork g_work;
+static MlpWork g_work, g_pos;
 
 static int scratch(int B, MlpScratch *sc, float **loss_part, float **grad) {
     const size_t need = (size_t)B * (6 * kH + kOut) + (size_t)(B / 256 + 2) + kNumParams;
@@ -388,19 +407,19 @@ extern "C" {
 int gsea_mlp_num_params(void) { return kNumParams; }
 
 int gsea_mlp_forward(const float *d_params, const float *d_x, int64_t B,
-                     int bf16, float *d_y, void *stream) {
+                     int bf16, const float *d_norm, float *d_y, void *stream) {
     if (B <= 0 || B > (1LL << 30) || !d_params || !d_x || !d_y)
         return set_error(GSEA_ERR_INVALID_ARGUMENT, "bad mlp forward arguments");
     hipStream_t st = (hipStream_t)stream;
     const int blocks = (int)((B + 255) / 256);
     if (bf16) {
         hipLaunchKernelGGL(k_mlp_infer_bf16, dim3(blocks), dim3(256), 0, st, d_params,
-                           d_x, (int)B, d_y);
+                           d_x, (int)B, d_y, d_norm);
     } else {
         MlpScratch sc{};
         hipLaunchKernelGGL(k_mlp_fwd_bwd<false>, dim3(blocks), dim3(256), 0, st,
                            d_params, d_x, (const float *)nullptr, (int)B, d_y, sc,
-                           (float *)nullptr);
+                           (float *)nullptr, d_norm);
     }
     SG_TRY(hipGetLastError());
     SG_TRY(hipStreamSynchronize(st));
@@ -418,7 +437,8 @@ int gsea_mlp_grad(const float *d_params, const float *d_x, const float *d_t,
     if (rc) return rc;
     const int blocks = (int)((B + 255) / 256);
     hipLaunchKernelGGL(k_mlp_fwd_bwd<true>, dim3(blocks), dim3(256), 0, st, d_params,
-                       d_x, d_t, (int)B, (float *)nullptr, sc, loss_part);
+                       d_x, d_t, (int)B, (float *)nullptr, sc, loss_part,
+                       (const float *)nullptr);
     SG_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_mlp_wgrad, dim3((kNumParams + 255) / 256), dim3(256), 0, st,
                        d_x, sc, (int)B, d_grad);
@@ -449,12 +469,24 @@ int gsea_mlp_adam(float *d_params, float *d_m, float *d_v, const float *d_grad,
     return GSEA_OK;
 }
 
-int gsea_set_features(const float *d_r0, const int32_t *d_pos0, int64_t N,
+int gsea_set_features(const float *d_r0, const int32_t *d_rank_genes, int64_t N,
                       const int32_t *d_set_offsets, const int32_t *d_set_genes,
                       int64_t G, float *d_glob, float *d_feat, void *stream) {
-    if (N <= 0 || G <= 0 || !d_r0 || !d_pos0 || !d_feat || !d_glob)
+    if (N <= 0 || N > GSEA_MAX_GENES || G <= 0 || !d_r0 || !d_rank_genes || !d_feat ||
+        !d_glob)
         return set_error(GSEA_ERR_INVALID_ARGUMENT, "bad feature arguments");
     hipStream_t st = (hipStream_t)stream;
+    if (g_pos.cap < (size_t)N) {
+        if (g_pos.buf) SG_TRY(hipFree(g_pos.buf));
+        g_pos.buf = nullptr;
+        g_pos.cap = 0;
+        SG_TRY(hipMalloc(&g_pos.buf, sizeof(int32_t) * GSEA_MAX_GENES));
+        g_pos.cap = GSEA_MAX_GENES;
+    }
+    int32_t *d_pos0 = reinterpret_cast<int32_t *>(g_pos.buf);
+    hipLaunchKernelGGL(k_invert_ranks, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                       d_rank_genes, (int)N, d_pos0);
+    SG_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_metric_summary, dim3(1), dim3(1024), 0, st, d_r0, (int)N, d_glob);
     SG_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_set_features, dim3((unsigned)((G + 3) / 4)), dim3(256), 0, st,

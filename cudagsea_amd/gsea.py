@@ -127,6 +127,37 @@ def run_device(X, labels, set_offsets, set_genes, num_perms, metric,
     return t.as_dict()
 
 
+def run_budget_device(X, labels, set_offsets, set_genes, num_perms, metric,
+                      budget, stats_out, es_pg_out=None, extension=None,
+                      ascending=False, swap_labels=False, stream=None):
+    """Budgeted run: set s is scored for permutations 0..budget[s]-1 (host
+    int array, each entry dividing num_perms).  extension = (fdr, z,
+    nes_tol) or None (see gsea_run_budget_device_f32).  Returns (final budgets, timings).  es_pg_out, if
+    given, is a [P][G] device tensor (entries beyond a set's budget are
+    unspecified)."""
+    S, N = X.shape
+    G = set_offsets.numel() - 1
+    double = X.dtype.itemsize == 8
+    labels = np.ascontiguousarray(labels, dtype=np.uint8)
+    bud = np.ascontiguousarray(budget, dtype=np.int32)
+    if bud.shape != (G,):
+        raise ValueError("budget must hold one entry per set")
+    out = np.zeros(G, dtype=np.int32)
+    ext = None
+    if extension is not None:
+        ext = _lib.gsea_extension(*[float(v) for v in extension])
+    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
+        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    t = _lib.gsea_timings()
+    fn = lib().gsea_run_budget_device_f64 if double else lib().gsea_run_budget_device_f32
+    st = ctypes.c_void_p(stream) if stream else None
+    check(fn(_dp(X), N, S, _ptr(labels), _dp(set_offsets), _dp(set_genes), G,
+             num_perms, metric_id(metric), flags, _ptr(bud),
+             ctypes.byref(ext) if ext is not None else None, _ptr(out),
+             _dp(es_pg_out), _dp(stats_out), st, ctypes.byref(t)))
+    return out, t.as_dict()
+
+
 def metric_device(X, labels, num_perms, metric, r_out, shift=0,
                   swap_labels=False, stream=None):
     """Stage (a): r_out[P][N] for global permutations shift..shift+P-1."""

@@ -9,6 +9,9 @@ permutations and the metric list.  numpy PCG64 with seed 20261203 + config id.
   c3: 20,000 x 500, 150/350, 10,000 sets log-uniform in [5, 2000], 16,384
   c4: 20,000 x 300, 150/150, N(1e4, 1e-2) per gene, 1% constant genes,
       2,000 sets of 15-500, 4,096 permutations
+  c5: c3's shape with its own seed, 400 shifted genes and 3% of the sets
+      carrying a planted 2-30% share of them (so that FDR < 0.25 calls
+      exist), 16,384 permutations, stable_signal2noise
 """
 import numpy as np
 
@@ -58,10 +61,18 @@ def _lognormal(rng, N, S, lab, shifted, shift):
 
 
 def make(config, dtype=np.float32, num_genes=None, num_samples=None,
-         num_sets=None, num_perms=None):
-    """Build one workload; the optional sizes shrink it for quick checks."""
-    cid = {"c1": 1, "c2": 2, "c3": 3, "c4": 4}[config]
-    rng = np.random.default_rng(SEED + cid)
+         num_sets=None, num_perms=None, seed=None):
+    """Build one workload; the optional sizes shrink it for quick checks.
+    c5 is c3's shape drawn with its own seed (the dataset the learned
+    permutation allocation is evaluated on); `seed` replaces the default
+    seed (fresh c3-shaped datasets for training the allocation model)."""
+    cid = {"c1": 1, "c2": 2, "c3": 3, "c4": 4, "c5": 5}[config]
+    rng = np.random.default_rng(SEED + cid if seed is None else seed)
+    if config == "c5":
+        config = "c3"
+        c5 = True
+    else:
+        c5 = False
     if config == "c1":
         N, S, nA, G, P = 2000, 40, 20, 50, 1024
         metrics = ["stable_signal2noise"]
@@ -86,7 +97,8 @@ def make(config, dtype=np.float32, num_genes=None, num_samples=None,
     nB = S - nA
     lab = _labels(rng, nA, nB)
     if config in ("c1", "c3"):
-        shifted = rng.choice(N, size=min(100, N), replace=False)
+        nshift = min(100 if not c5 else 400, N)
+        shifted = rng.choice(N, size=nshift, replace=False)
         X = _lognormal(rng, N, S, lab, shifted, 0.5)
     elif config == "c2":
         shifted = rng.choice(N, size=max(1, N // 20), replace=False)
@@ -102,9 +114,19 @@ def make(config, dtype=np.float32, num_genes=None, num_samples=None,
     lo = min(15 if config != "c3" else 5, hi)
     if config == "c3":
         sets = _sets_loguniform(rng, G, N, lo, hi)
+        if c5:
+            # 3% of the sets carry a planted share (2-30%) of shifted genes
+            for s in rng.choice(G, size=max(1, (3 * G) // 100), replace=False):
+                k = sets[s].size
+                m = min(shifted.size, max(1, int(round(k * rng.uniform(0.02, 0.3)))))
+                keep = rng.choice(sets[s], size=k - m, replace=False)
+                add = rng.choice(shifted, size=m, replace=False)
+                sets[s] = np.unique(np.concatenate([keep, add]))
     else:
         sets = _sets_uniform(rng, G, N, lo, hi)
     off, genes = _csr(sets)
+    if c5:
+        config, metrics = "c5", ["stable_signal2noise"]
     return {
         "config": config, "X": np.ascontiguousarray(X, dtype=dtype),
         "labels": lab, "set_offsets": off, "set_genes": genes,

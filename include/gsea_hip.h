@@ -111,6 +111,47 @@ int gsea_run_device_f64(const double *d_exprs, int64_t num_genes,
                         double *d_es_out, double *d_stats_out, void *stream,
                         gsea_timings *timings);
 
+/* Budgeted run (no counterpart in the reference; used by the learned
+ * permutation allocation).  Set s is scored for permutations
+ * 0..budget[s]-1 only; every budget divides num_perms and is >= 2.  Its
+ * NES, NP and FWER use those permutations, and the FDR counts each of its
+ * null scores num_perms/budget[s] times.  With `ext`, the sets whose
+ * significance call is not settled, or whose NES is imprecise, are then
+ * scored for all permutations and the statistics recomputed; with se = a
+ * set's NES standard error (coefficient of variation of |null ES| times
+ * |NES| over sqrt(budget)):
+ *   call rule (fdr > 0): the sets of the same NES sign with NES within
+ *     z*se of the set's (itself included) have FDR values on both sides of
+ *     fdr (or equal to it);
+ *   precision rule (nes_tol > 0): z*se > nes_tol.
+ * budget_out (host, may be NULL) receives the final budgets; d_es_pg (may
+ * be NULL) receives es[P][G], whose entries beyond a set's budget are
+ * unspecified.  With every budget equal to num_perms the results equal
+ * gsea_run_device_*'s. */
+typedef struct gsea_extension {
+    double fdr;       /* significance threshold of the call rule */
+    double z;         /* standard errors of NES uncertainty */
+    double nes_tol;   /* NES precision target */
+} gsea_extension;
+int gsea_run_budget_device_f32(const float *d_exprs, int64_t num_genes,
+                               int64_t num_samples, const uint8_t *labels,
+                               const int32_t *d_set_offsets,
+                               const int32_t *d_set_genes, int64_t num_sets,
+                               int64_t num_perms, int metric_id, int flags,
+                               const int32_t *budget, const gsea_extension *ext,
+                               int32_t *budget_out, float *d_es_pg,
+                               float *d_stats_out, void *stream,
+                               gsea_timings *timings);
+int gsea_run_budget_device_f64(const double *d_exprs, int64_t num_genes,
+                               int64_t num_samples, const uint8_t *labels,
+                               const int32_t *d_set_offsets,
+                               const int32_t *d_set_genes, int64_t num_sets,
+                               int64_t num_perms, int metric_id, int flags,
+                               const int32_t *budget, const gsea_extension *ext,
+                               int32_t *budget_out, double *d_es_pg,
+                               double *d_stats_out, void *stream,
+                               gsea_timings *timings);
+
 /* Stage entry points (device pointers), for tests and callers that hold
  * intermediate results.  Replace correllate_genes_gpu
  * (correlate_genes.cuh:121-300) and pathway_sort_gpu + compute_scores_gpu
@@ -183,9 +224,12 @@ int gsea_proxy_run(int double_precision, const void *exprs, int64_t num_genes,
 #define GSEA_MLP_NUM_PARAMS 5508
 
 int gsea_mlp_num_params(void);
-/* y[B][4] = mlp(x[B][16]); bf16 != 0 rounds weights and activations to bf16 */
+/* y[B][4] = mlp(x[B][16]); bf16 != 0 rounds weights and activations to
+ * bf16.  d_norm (may be NULL): 40 floats x_mean[16] x_inv_std[16]
+ * y_scale[4] y_shift[4]; inputs become (x - x_mean) * x_inv_std (in fp32,
+ * before any bf16 rounding) and outputs y * y_scale + y_shift. */
 int gsea_mlp_forward(const float *d_params, const float *d_x, int64_t batch,
-                     int bf16, float *d_y, void *stream);
+                     int bf16, const float *d_norm, float *d_y, void *stream);
 /* gradient of mean((mlp(x) - t)^2) over batch x 4 outputs; *loss (host) */
 int gsea_mlp_grad(const float *d_params, const float *d_x, const float *d_t,
                   int64_t batch, float *d_grad, float *loss, void *stream);
@@ -193,9 +237,10 @@ int gsea_mlp_grad(const float *d_params, const float *d_x, const float *d_t,
  * the gradient); step counts from 1 */
 int gsea_mlp_adam(float *d_params, float *d_m, float *d_v, const float *d_grad,
                   int64_t step, float lr, float weight_decay, void *stream);
-/* features[G][16] of every set from the observed metric r0[N] and observed
- * ranks pos0[N]; glob[4] receives summaries of r0 */
-int gsea_set_features(const float *d_r0, const int32_t *d_pos0, int64_t num_genes,
+/* features[G][16] of every set from the observed metric r0[N] and the
+ * observed ranking rank_genes[N] (gene at each rank, gsea_argsort_f32);
+ * glob[4] receives summaries of r0 */
+int gsea_set_features(const float *d_r0, const int32_t *d_rank_genes, int64_t num_genes,
                       const int32_t *d_set_offsets, const int32_t *d_set_genes,
                       int64_t num_sets, float *d_glob, float *d_feat,
                       void *stream);

@@ -1,0 +1,308 @@
+"""Learned permutation allocation on the GPU: the MLP kernels against the
+float64 numpy network of oracle/mlp_ref.py (forward, loss, every parameter's
+gradient, Adam), features and null moments against numpy restatements, the
+budgeted engine run against the plain run, and training on engine-generated
+null moments (the network beats the size-only allocation model).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mlp_ref
+from cudagsea_amd import surrogate as sg
+from cudagsea_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_mlp_forward_matches_reference(gpu):
+    rng = np.random.default_rng(3)
+    params = mlp_ref.init_params(5) + 0.01 * rng.standard_normal(mlp_ref.NUM_PARAMS).astype(np.float32)
+    x = rng.standard_normal((300, mlp_ref.IN)).astype(np.float32)
+    y = sg.mlp_forward(dev(params), dev(x)).cpu().numpy()
+    want, _ = mlp_ref.forward(params.astype(np.float64), x.astype(np.float64))
+    np.testing.assert_allclose(y, want, rtol=1e-4, atol=1e-5)
+
+
+def test_mlp_forward_standardisation(gpu):
+    """The optional in-kernel standardisation: inputs (x - mean) * inv_std,
+    outputs y * scale + shift, in both the fp32 and the bf16 kernel."""
+    rng = np.random.default_rng(12)
+    params = mlp_ref.init_params(13)
+    x = (3.0 + 2.0 * rng.standard_normal((500, mlp_ref.IN))).astype(np.float32)
+    mean = rng.standard_normal(16).astype(np.float32)
+    inv = (0.5 + rng.random(16)).astype(np.float32)
+    scale = (0.2 + rng.random(4)).astype(np.float32)
+    shift = rng.standard_normal(4).astype(np.float32)
+    norm = np.concatenate([mean, inv, scale, shift])
+    xs = (x.astype(np.float64) - mean) * inv
+    want, _ = mlp_ref.forward(params.astype(np.float64), xs)
+    want = want * scale + shift
+    y = sg.mlp_forward(dev(params), dev(x), norm=dev(norm)).cpu().numpy()
+    np.testing.assert_allclose(y, want, rtol=1e-4, atol=1e-4)
+    y16 = sg.mlp_forward(dev(params), dev(x), bf16=True, norm=dev(norm)).cpu().numpy()
+    assert np.abs(y16 - want).max() <= 0.05 * np.abs(want).max()
+
+
+def test_mlp_bf16_inference_close_to_fp32(gpu):
+    rng = np.random.default_rng(4)
+    params = mlp_ref.init_params(6)
+    x = rng.standard_normal((1000, mlp_ref.IN)).astype(np.float32)
+    y32 = sg.mlp_forward(dev(params), dev(x)).cpu().numpy()
+    y16 = sg.mlp_forward(dev(params), dev(x), bf16=True).cpu().numpy()
+    # bf16 weights/activations: 8-bit significands, fp32 accumulation
+    scale = np.abs(y32).max()
+    assert np.abs(y16 - y32).max() <= 0.05 * scale
+    assert np.abs(y16 - y32).mean() <= 0.01 * scale
+
+
+@pytest.mark.parametrize("B", [1, 257, 2048])
+def test_mlp_gradient_matches_reference(gpu, B):
+    """Every one of the 5508 parameter gradients and the loss against the
+    float64 reference (analytic backprop, itself checked against finite
+    differences in the CPU suite)."""
+    rng = np.random.default_rng(B)
+    params = mlp_ref.init_params(B + 1)
+    x = rng.standard_normal((B, mlp_ref.IN)).astype(np.float32)
+    t = rng.standard_normal((B, mlp_ref.OUT)).astype(np.float32)
+    g, loss = sg.mlp_grad(dev(params), dev(x), dev(t))
+    wl, wg = mlp_ref.loss_and_grad(params.astype(np.float64), x.astype(np.float64),
+                                   t.astype(np.float64))
+    assert abs(loss - wl) <= 1e-5 * abs(wl) + 1e-7
+    g = g.cpu().numpy().astype(np.float64)
+    scale = np.abs(wg).max()
+    assert np.abs(g - wg).max() <= 2e-5 * scale + 1e-7
+    # per-tensor relative error (no block of the network may be off)
+    o = 0
+    for name, shape in mlp_ref.SHAPES:
+        n = int(np.prod(shape))
+        a, b = g[o:o + n], wg[o:o + n]
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-7, name
+        o += n
+
+
+def test_adam_step_matches_numpy(gpu):
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(sg.NUM_PARAMS).astype(np.float32)
+    m = (0.1 * rng.standard_normal(sg.NUM_PARAMS)).astype(np.float32)
+    v = np.abs(0.1 * rng.standard_normal(sg.NUM_PARAMS)).astype(np.float32)
+    g = rng.standard_normal(sg.NUM_PARAMS).astype(np.float32)
+    dp, dm, dv = dev(p), dev(m), dev(v)
+    step, lr, wd = 7, 1e-3, 1e-4
+    sg.mlp_adam(dp, dm, dv, dev(g), step, lr, wd)
+    gg = g.astype(np.float64) + wd * p
+    m2 = 0.9 * m + 0.1 * gg
+    v2 = 0.999 * v + 0.001 * gg * gg
+    c1, c2 = 1 - 0.9 ** step, 1 - 0.999 ** step
+    p2 = p - lr * (m2 / c1) / (np.sqrt(v2 / c2) + 1e-8)
+    # fp32 kernel vs float64 numpy: the constants (1 - beta) alone differ by
+    # ~5e-5 relative in fp32
+    np.testing.assert_allclose(dm.cpu().numpy(), m2, rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(dv.cpu().numpy(), v2, rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(dp.cpu().numpy(), p2, rtol=1e-5, atol=1e-6)
+
+
+def numpy_features(r0, pos0, off, genes):
+    N = r0.size
+    r0 = r0.astype(np.float64)
+    m = r0.mean()
+    sd = math.sqrt(max((r0 * r0).mean() - m * m, 0.0))
+    gabs = np.abs(r0).mean()
+    gmax = np.abs(r0).max()
+    gskew = ((r0 ** 3).mean() - 3 * m * sd * sd - m ** 3) / sd ** 3 if sd > 0 else 0.0
+    out = []
+    for s in range(off.size - 1):
+        g = genes[off[s]:off[s + 1]]
+        k = max(g.size, 1)
+        r, a, q = r0[g], np.abs(r0[g]), pos0[g] / N
+        mr = r.sum() / k
+        vr = max((r * r).sum() / k - mr * mr, 0.0)
+        sdr = math.sqrt(vr)
+        skew = ((r ** 3).sum() / k - 3 * mr * vr - mr ** 3) / sdr ** 3 if sdr > 0 else 0.0
+        ma = a.sum() / k
+        va = max((a * a).sum() / k - ma * ma, 0.0)
+        mq = q.sum() / k
+        vq = max((q * q).sum() / k - mq * mq, 0.0)
+        out.append([math.log(k) / math.log(N), k / N, mr / sd, sdr / sd,
+                    min(max(skew, -10), 10), ma / gabs, math.sqrt(va) / gabs, mq,
+                    math.sqrt(vq), (q < 0.1).sum() / k, (q < 0.25).sum() / k,
+                    (q >= 0.75).sum() / k, (q >= 0.9).sum() / k,
+                    (a.max() if g.size else 0.0) / gmax, sd / gabs,
+                    min(max(gskew, -10), 10)])
+    return np.array(out)
+
+
+def test_set_features_match_numpy(gpu):
+    import torch
+    w = workloads.make("c3", num_genes=3000, num_samples=60, num_sets=40, seed=11)
+    Xd = dev(w["X"])
+    r0, rg = sg.observed_ranking(Xd, w["labels"], "stable_signal2noise")
+    feat = sg.set_features(r0, rg, dev(w["set_offsets"]), dev(w["set_genes"])).cpu().numpy()
+    r0h = r0.cpu().numpy()
+    rgh = rg.cpu().numpy()
+    # the ranking is a permutation; descending metric, ties by gene id
+    assert np.array_equal(np.sort(rgh), np.arange(r0h.size))
+    order = np.lexsort((np.arange(r0h.size), -r0h.astype(np.float64)))
+    assert np.array_equal(rgh, order)
+    pos0 = np.empty(r0h.size, np.int64)
+    pos0[rgh] = np.arange(r0h.size)
+    want = numpy_features(r0h, pos0, w["set_offsets"], w["set_genes"])
+    np.testing.assert_allclose(feat, want, rtol=2e-5, atol=2e-6)
+    del torch
+
+
+def test_null_moments_match_numpy(gpu):
+    rng = np.random.default_rng(2)
+    P, G = 700, 33
+    es = rng.standard_normal((P, G)).astype(np.float32) * 0.3
+    es[:, 5] = np.abs(es[:, 5])          # one set with no negative null ES
+    got = sg.null_moments(dev(es)).cpu().numpy()
+    for s in range(G):
+        v = es[1:, s].astype(np.float64)
+        pos, neg = v[v > 0], -v[v <= 0]
+        want = [pos.mean() if pos.size else 0.0,
+                pos.std() if pos.size > 1 else 0.0,
+                neg.mean() if neg.size else 0.0,
+                neg.std() if neg.size > 1 else 0.0,
+                pos.size / v.size]
+        np.testing.assert_allclose(got[s], want, rtol=1e-4, atol=1e-6)
+
+
+# ------------------------------------------------------ budgeted run ----
+def _plain(w, P, metric):
+    import torch
+    G = w["G"]
+    stats = torch.empty((5, G), dtype=torch.float32, device="cuda")
+    es = torch.empty((G, P), dtype=torch.float32, device="cuda")
+    import cudagsea_amd as cg
+    cg.run_device(dev(w["X"]), w["labels"], dev(w["set_offsets"]), dev(w["set_genes"]),
+                  P, metric, stats, es_out=es)
+    return stats.cpu().numpy(), es.cpu().numpy()
+
+
+def _budget(w, P, metric, budget, ext=None):
+    import torch
+    import cudagsea_amd as cg
+    G = w["G"]
+    stats = torch.empty((5, G), dtype=torch.float32, device="cuda")
+    es = torch.empty((P, G), dtype=torch.float32, device="cuda")
+    bfin, _ = cg.run_budget_device(dev(w["X"]), w["labels"], dev(w["set_offsets"]),
+                                   dev(w["set_genes"]), P, metric, budget, stats,
+                                   es_pg_out=es, extension=ext)
+    return stats.cpu().numpy(), es.cpu().numpy(), bfin
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def test_budget_all_full_equals_plain(gpu):
+    w = workloads.make("c3", num_genes=2500, num_samples=80, num_sets=120, seed=21)
+    P = 512
+    sp, _ = _plain(w, P, "onepass_t_test")
+    sb, _, bfin = _budget(w, P, "onepass_t_test", np.full(w["G"], P, np.int32))
+    assert np.array_equal(bits(sp), bits(sb))
+    assert (bfin == P).all()
+
+
+def test_budget_statistics_use_each_sets_permutations(gpu):
+    w = workloads.make("c3", num_genes=2500, num_samples=80, num_sets=150, seed=22)
+    P = 1024
+    G = w["G"]
+    rng = np.random.default_rng(1)
+    budget = rng.choice([64, 256, 1024], size=G).astype(np.int32)
+    sp, es_plain = _plain(w, P, "stable_signal2noise")
+    sb, es_b, bfin = _budget(w, P, "stable_signal2noise", budget)
+    assert np.array_equal(bfin, budget)
+    # every scored entry equals the plain run's ES bit for bit
+    for s in range(G):
+        B = budget[s]
+        assert np.array_equal(bits(es_b[:B, s]), bits(es_plain[s, :B])), s
+    # NES / NP from the set's own permutations (sequential fp32 sum of |ES|)
+    for s in range(G):
+        B = budget[s]
+        v = np.abs(es_plain[s, :B]).astype(np.float32)
+        tot = np.float32(0)
+        for x in v:
+            tot = np.float32(tot + x)
+        e0 = es_plain[s, 0]
+        f = np.float32(np.float32(B) / tot)
+        assert bits(sb[1, s]) == bits(np.float32(e0 * f)), s
+        c = int((v >= abs(e0)).sum())
+        assert bits(sb[2, s]) == bits(np.float32(np.float32(c) / np.float32(B))), s
+    # full-budget sets keep the plain run's ES, NES and NP
+    full = budget == P
+    assert np.array_equal(bits(sp[:3, full]), bits(sb[:3, full]))
+    fdr = sb[4]
+    assert np.all((fdr >= 0) & (fdr <= 1) | np.isnan(fdr))
+
+
+def test_budget_extension_to_all_permutations(gpu):
+    w = workloads.make("c3", num_genes=2500, num_samples=80, num_sets=100, seed=23)
+    P = 1024
+    sp, _ = _plain(w, P, "twopass_t_test")
+    budget = np.full(w["G"], 128, np.int32)
+    # nes_tol tiny: every set's NES precision misses it -> all get P
+    sb, _, bfin = _budget(w, P, "twopass_t_test", budget, ext=(0.0, 4.0, 1e-9))
+    assert (bfin == P).all()
+    assert np.array_equal(bits(sp), bits(sb))
+    # call rule only: the sets extended are exactly those whose NES window
+    # (z standard errors, same-sign sets) holds FDR values on both sides
+    thr, z = 0.3, 3.0
+    sb2, es2, bfin2 = _budget(w, P, "twopass_t_test", budget, ext=(thr, z, 0.0))
+    s1, es1, _ = _budget(w, P, "twopass_t_test", budget)
+    nes, fdr = s1[1].astype(np.float64), s1[4].astype(np.float64)
+    a = np.abs(es1[:128]).astype(np.float64)
+    cv = a.std(0) / a.mean(0)
+    se = cv * np.abs(nes) / np.sqrt(128)
+    want = np.zeros(w["G"], bool)
+    for s in range(w["G"]):
+        same = np.sign(nes) == np.sign(nes[s])
+        win = same & (np.abs(nes - nes[s]) <= z * se[s]) & np.isfinite(fdr)
+        f = np.append(fdr[win], fdr[s])
+        want[s] = (f <= thr).any() and (f >= thr).any()
+    got = bfin2 == P
+    # the window edges use the kernel's fp32 cv; allow sets within 1e-4
+    # relative of an edge to differ
+    assert (got != want).sum() <= 2, (got.sum(), want.sum())
+    assert got.any() or not want.any()
+
+
+def test_budget_rejects_bad_budgets(gpu):
+    import cudagsea_amd as cg
+    w = workloads.make("c3", num_genes=500, num_samples=40, num_sets=10, seed=24)
+    for bad in (np.full(10, 3, np.int32), np.full(10, 1, np.int32),
+                np.full(10, 2048, np.int32)):
+        with pytest.raises(cg.GseaError):
+            _budget(w, 1024, "onepass_t_test", bad)
+
+
+# ------------------------------------------------------------ training ----
+def test_training_learns_null_moments_beyond_set_size(gpu):
+    """Engine-generated null moments of small c3-shaped datasets: the
+    network's held-out error is below the size-only model's, and its loss
+    falls during training."""
+    feats, tg = sg.make_training_data(8, perms=512, seed=500, num_sets=1500,
+                                      metrics=["naive_diff_of_classes",
+                                               "stable_signal2noise"])
+    assert feats.shape == (8 * 2 * 1500, 16) and tg.shape == (8 * 2 * 1500, 4)
+    assert np.isfinite(feats).all() and np.isfinite(tg).all()
+    rng = np.random.default_rng(0)
+    idx = rng.permutation(len(feats))
+    te, tr = idx[:3000], idx[3000:]
+    model, hist = sg.train(feats[tr], tg[tr], epochs=20, batch=512, lr=3e-3)
+    assert hist["val"][-1] < 0.5 * hist["val"][0]
+    import torch
+    pred = np.log(model.predict(torch.from_numpy(feats[te]).cuda(), bf16=True))
+    coef = sg.size_baseline(feats[tr], tg[tr])
+    base = np.log(sg.size_baseline_predict(coef, feats[te]))
+    mse_net = ((pred - tg[te]) ** 2).mean()
+    mse_size = ((base - tg[te]) ** 2).mean()
+    print("held-out log-moment MSE: network %.4g, size-only %.4g" % (mse_net, mse_size))
+    assert mse_net < 0.8 * mse_size
