@@ -7,7 +7,10 @@ metrics -- the ragged batched configuration.  One step = the full GSEA path
 each of the 14 metrics, inputs resident in HBM.  value = ES/s = 14 * G * P *
 steps / time.
 
-Prints one JSON line with roofline and cpu_baseline objects.
+A second leg times the surrogate-guided run against the plain full run on
+c5 (identical FDR < 0.25 calls and max |dNES| < 0.02 are checked and
+reported).  Prints one JSON line with roofline, cpu_baseline and surrogate
+objects.
 """
 import argparse
 import json
@@ -23,15 +26,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(kernel, N, S, G, P, sumk, w):
-    """Per-launch algorithmic HBM bytes, materialised-stage model:
-    metric:  P*(N*w + S) + N*S*w          (write r, read labels; X once)
-    rank_es: P*(N*(3w+8) + G*w) + N*G/8 + 4*sumk
-             (read r, write+read ranked keys and 4-byte ids, write ES;
-              membership once)."""
-    if kernel == "metric":
+def algorithmic_bytes(kernel, N, S, G, P, w):
+    """Algorithmic HBM bytes of one launch over P permutations, the stage
+    model with materialised stage boundaries:
+      k_metric: P*(N*w + S) + N*S*w   (write r, read labels; X once)
+      k_es:     P*(N*(w+4) + G*w) + N*G/8
+                (read the ranked list: |metric| and a 4-byte id per rank;
+                 write ES; set membership bits once)."""
+    if kernel == "k_metric":
         return P * (N * w + S) + N * S * w
-    return P * (N * (3 * w + 8) + G * w) + N * G / 8 + 4 * sumk
+    return P * (N * (w + 4) + G * w) + N * G / 8
+
+
+def path_bytes_per_es(N, S, G, P, w):
+    """Whole-path algorithmic bytes per enrichment score (all stages)."""
+    return (P * (N * (4 * w + 8) + S + G * w) + N * S * w + N * G / 8) / (G * P)
 
 
 def cpu_baseline(w, metric, target_s):
@@ -58,6 +67,33 @@ def cpu_baseline(w, metric, target_s):
                       % (metric, w["N"], w["S"], w["G"], P1, w["P"], dt, cores)}
 
 
+def surrogate_leg(args):
+    """Surrogate-guided vs plain full run on c5 (c3's shape, its own seed,
+    planted enriched sets, 16,384 permutations, stable_signal2noise): both
+    end to end from resident inputs, best of 2 after a warm-up."""
+    from cudagsea_amd import surrogate as sg
+    from cudagsea_amd import workloads
+    if not os.path.exists(sg.MODEL_PATH):
+        return {"skipped": "no trained model at %s" % sg.MODEL_PATH}
+    print("surrogate leg (c5) ...", file=sys.stderr, flush=True)
+    model = sg.Surrogate.load()
+    w = workloads.make("c5", dtype=np.float64 if args.double else np.float32)
+    res, plain = sg.evaluate(w, model)
+    coef = model.meta["size_baseline"]
+    abl, _ = sg.evaluate(w, model, plain=plain,
+                         predictor=lambda f: sg.size_baseline_predict(coef, f))
+    out = {k: res[k] for k in ("speedup", "s_plain", "s_adaptive", "calls_plain",
+                               "calls_adaptive", "calls_identical", "max_abs_dnes",
+                               "es_work_fraction", "sets_extended", "budget_hist",
+                               "metric", "num_perms", "num_sets")}
+    out["workload"] = ("c5: %d genes x %d samples, %d ragged sets (3%% planted), "
+                       "%d permutations" % (w["N"], w["S"], w["G"], w["P"]))
+    out["bar_met"] = bool(res["calls_identical"] and res["max_abs_dnes"] < 0.02)
+    out["size_only_model"] = {k: abl[k] for k in ("speedup", "calls_identical",
+                                                  "max_abs_dnes", "es_work_fraction")}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +106,24 @@ def main():
     ap.add_argument("--double", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate-guided vs plain comparison (c5)")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES on the device (counter calibration)")
     args = ap.parse_args()
+
+    if args.probe_copy:
+        # calibration for HBM byte counters: one device-to-device copy of a
+        # known size (read + write of probe_copy bytes each)
+        import torch
+        n = args.probe_copy // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe_copy_bytes": n * 4}), flush=True)
+        return
 
     if args.gpus != 1:
         print("bench.py runs on one GPU only: multi-GPU permutation sharding is "
@@ -88,7 +141,6 @@ def main():
                        num_perms=args.perms if args.perms else None)
     metrics = w["metrics"] if args.metrics == "all" else args.metrics.split(",")
     N, S, G, P = w["N"], w["S"], w["G"], w["P"]
-    sumk = int(w["set_offsets"][-1])
     X = torch.from_numpy(w["X"]).cuda()
     off = torch.from_numpy(w["set_offsets"]).cuda()
     genes = torch.from_numpy(w["set_genes"]).cuda()
@@ -119,33 +171,42 @@ def main():
     es_total = float(G) * P * n_runs
     value = es_total / elapsed
     ms_metric = sum(t["ms_metric"] for t in all_t)
-    ms_rank = sum(t["ms_rank_es"] for t in all_t)
+    ms_rank = sum(t["ms_rank"] for t in all_t)
+    ms_es = sum(t["ms_es"] for t in all_t)
     ms_stats = sum(t["ms_stats"] for t in all_t)
     batches = sum(t["perm_batches"] for t in all_t)
+    es_launches = sum(t["es_launches"] for t in all_t)
     wbytes = np.dtype(dt).itemsize
-    # dominant kernel and its roofline
-    if ms_rank >= ms_metric:
-        kern, ms, name = "rank_es", ms_rank, "k_rank_es"
+    # dominant kernel: its launches' average duration (HIP events on the
+    # engine's stream, inside the timed region) against its algorithmic bytes
+    if ms_es >= ms_metric:
+        name, ms, launches = "k_es", ms_es, es_launches
     else:
-        kern, ms, name = "metric", ms_metric, "k_metric"
-    per_launch_ms = ms / batches
-    per_launch_P = P / (batches / n_runs)
-    abytes = algorithmic_bytes(kern, N, S, G, per_launch_P, sumk, wbytes)
+        name, ms, launches = "k_metric", ms_metric, batches
+    per_launch_ms = ms / launches
+    per_launch_P = P * n_runs / launches
+    abytes = algorithmic_bytes(name, N, S, G, per_launch_P, wbytes)
     achieved = abytes / (per_launch_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (name, args.config))
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
+            rec = json.load(open(tpath))
+            if int(rec.get("permutations", -1)) == int(per_launch_P):
+                traffic = rec.get("bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+    bpe = path_bytes_per_es(N, S, G, P, wbytes)
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic, "kernel": name,
             "launch_ms": round(per_launch_ms, 4),
-            "algorithmic_bytes": int(abytes)}
+            "algorithmic_bytes": int(abytes),
+            "path_bytes_per_es": round(bpe, 3),
+            "path_frac": round(bpe * value / (HBM_PEAK_GBS * 1e9), 6)}
     stage_ms = {"metric": round(ms_metric / n_runs, 3),
-                "rank_es": round(ms_rank / n_runs, 3),
+                "rank": round(ms_rank / n_runs, 3),
+                "es": round(ms_es / n_runs, 3),
                 "stats": round(ms_stats / n_runs, 3)}
 
     cpu = None
@@ -153,8 +214,13 @@ def main():
         print("cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w, metrics[len(metrics) // 2], args.cpu_seconds)
 
+    sur = None
+    if not args.no_surrogate:
+        sur = surrogate_leg(args)
+
     line = {
-        "metric": "ES/sec (gene-set x perm) at 20k genes, 1 GPU",
+        "metric": "ES/sec (gene-set\u00d7perm) at 20k genes, 1 GPU; surrogate vs "
+                  "plain at equal FDR",
         "value": value, "unit": "ES/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -165,7 +231,7 @@ def main():
                    "genes": N, "samples": S, "gene_sets": G,
                    "permutations": P, "metrics": len(metrics),
                    "mean_ms_per_metric_run": stage_ms},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 

@@ -40,7 +40,8 @@ class gsea_timings(ctypes.Structure):
     _fields_ = [("ms_lists", ctypes.c_double), ("ms_metric", ctypes.c_double),
                 ("ms_rank_es", ctypes.c_double), ("ms_stats", ctypes.c_double),
                 ("ms_transpose", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("perm_batches", ctypes.c_int64)]
+                ("perm_batches", ctypes.c_int64), ("ms_rank", ctypes.c_double),
+                ("ms_es", ctypes.c_double), ("es_launches", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -168,14 +168,17 @@ static int finish_timings(StageClock &clk, gsea_timings *t, int64_t batches) {
     std::memset(t, 0, sizeof(*t));
     t->perm_batches = batches;
     if (clk.marks.empty()) return GSEA_OK;
-    double *slots[5] = {&t->ms_lists, &t->ms_metric, &t->ms_rank_es,
-                        &t->ms_stats, &t->ms_transpose};
+    // stages: 0 lists, 1 metric, 2 rank, 3 statistics, 4 transpose, 5 ES
+    double *slots[6] = {&t->ms_lists, &t->ms_metric, &t->ms_rank,
+                        &t->ms_stats, &t->ms_transpose, &t->ms_es};
     for (auto &m : clk.marks) {
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, clk.d->events[m.second.first],
                                     clk.d->events[m.second.second]));
         *slots[m.first] += ms;
+        if (m.first == 5) ++t->es_launches;
     }
+    t->ms_rank_es = t->ms_rank + t->ms_es;
     float tot = 0.f;
     HIP_TRY(hipEventElapsedTime(&tot, clk.d->events[clk.marks.front().second.first],
                                 clk.d->events[clk.marks.back().second.second]));
@@ -716,7 +719,12 @@ static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
         if ((rc = clk.end())) return rc;
         if ((rc = clk.begin(2))) return rc;
         rc = launch_rank<T>((const T *)pr, N, pb, base, d_off, d_genes, G, flags,
-                            (T *)pes, d, st, &tiers);
+                            (T *)pes, d, st, nullptr);
+        if (rc) return rc;
+        if ((rc = clk.end())) return rc;
+        if ((rc = clk.begin(5))) return rc;
+        rc = launch_es_rows<T>(N, pb, base, d_off, d_genes, G, flags, (T *)pes, d, st,
+                               &tiers, 0, pb);
         if (rc) return rc;
         if ((rc = clk.end())) return rc;
     }
@@ -873,15 +881,17 @@ static int run_device_budget(const T *dX, int64_t N, int64_t S, const uint8_t *l
     auto score = [&](int64_t base, int64_t pb, const std::vector<int64_t> &lo,
                      const std::vector<int64_t> &hi, std::vector<SetTiers> &tv) -> int {
         int r;
-        if ((r = clk.begin(2))) return r;
         for (size_t l = 0; l < tv.size(); ++l) {
             const int64_t a = std::max(lo[l], base) - base;
             const int64_t b = std::min(hi[l], base + pb) - base;
+            if (b <= a) continue;
+            if ((r = clk.begin(5))) return r;
             r = launch_es_rows<T>(N, Pb, base, d_off, d_genes, G, flags, (T *)pes, d, st,
                                   &tv[l], a, b);
             if (r) return r;
+            if ((r = clk.end())) return r;
         }
-        return clk.end();
+        return GSEA_OK;
     };
 
     int64_t batches = 0, resident = -1;

@@ -55,11 +55,14 @@ extern "C" {
 typedef struct gsea_timings {
     double ms_lists;      /* permutation class lists */
     double ms_metric;     /* local ranking metric */
-    double ms_rank_es;    /* argsort + enrichment scores */
+    double ms_rank_es;    /* argsort + enrichment scores (ms_rank + ms_es) */
     double ms_stats;      /* NES, NP, FWER, FDR */
     double ms_transpose;  /* [P][G] -> [G][P] when es_out is requested */
     double ms_total;      /* first launch to last completion */
     int64_t perm_batches; /* permutation batches used */
+    double ms_rank;       /* argsort (rank rows) */
+    double ms_es;         /* enrichment-score kernel launches */
+    int64_t es_launches;  /* enrichment-score kernel launches timed */
 } gsea_timings;
 
 /* metric name <-> id; ids follow the dispatch order of
