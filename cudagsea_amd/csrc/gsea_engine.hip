@@ -66,7 +66,7 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kNumBufs
+    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -79,6 +79,7 @@ struct DeviceState {
     int cu_count = 256;
     Buffer bufs[kNumBufs];
     Buffer tier_bufs[kTierSlots][kNumTierBufs];
+    int64_t rcp_len = 0;   // entries of the 1/(i+1) table in kBufRcp
     std::vector<hipEvent_t> events;
 };
 
@@ -226,12 +227,14 @@ static hipError_t launch_metric_gt(const MetricArgs<T> &a, bool xlds,
                                    hipStream_t st) {
     const int gx = (a.N + GT - 1) / GT;
     const int gy = (a.Pb + a.ppb - 1) / a.ppb;
+    // fp32 with the X tile in LDS: two genes per lane
+    constexpr int GPL = sizeof(T) == 4 ? 2 : 1;
     if (xlds) {
         const size_t lds = (size_t)a.S * GT * sizeof(T);
-        hipLaunchKernelGGL((k_metric<T, KIND, GT, true>), dim3(gx, gy), dim3(256),
+        hipLaunchKernelGGL((k_metric<T, KIND, GT, true, GPL>), dim3(gx, gy), dim3(256),
                            lds, st, a);
     } else {
-        hipLaunchKernelGGL((k_metric<T, KIND, GT, false>), dim3(gx, gy), dim3(256),
+        hipLaunchKernelGGL((k_metric<T, KIND, GT, false, 1>), dim3(gx, gy), dim3(256),
                            0, st, a);
     }
     return hipGetLastError();
@@ -266,6 +269,17 @@ static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
     a.lstride = L.lstride; a.nA = L.nA; a.nB = L.nB; a.offB = L.offB;
     a.lenA = L.lenA;
     a.Pb = (int)Pb; a.metric = metric; a.r = r;
+    // 1/(i+1) for the Welford running counts, correctly rounded doubles
+    if (d->rcp_len < S) {
+        std::vector<double> rcp((size_t)S);
+        for (int64_t i = 0; i < S; ++i) rcp[(size_t)i] = 1.0 / (double)(i + 1);
+        void *pr;
+        int rc = ensure(d, kBufRcp, sizeof(double) * S, &pr);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(pr, rcp.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+        d->rcp_len = S;
+    }
+    a.rcp = (const double *)d->bufs[kBufRcp].ptr;
     bool xlds = false;
     const int gt = metric_tile<T>(S, &xlds);
     const int64_t gx = (N + gt - 1) / gt;

@@ -54,7 +54,9 @@ __global__ void k_class_lists(const uint8_t *__restrict__ labels, int S,
 //
 // Block = 4 wavefronts.  The block stages X[0:S][g0:g0+GT] (sample-major
 // input) in LDS once and streams `ppb` permutations over it.  A wavefront is
-// split into 64/GT lane groups; lane `col` of a group owns gene g0+col, and
+// split into lane groups of GT/GPL lanes; lane `col` of a group owns genes
+// g0+col*GPL .. +GPL-1 (GPL = 2 for fp32: one 8-byte LDS read per sample
+// serves both, so the class-list decoding and addressing is shared), and
 // each lane carries PL permutations at once (independent accumulation
 // chains).  Every class sum walks that class's samples in increasing sample
 // order, one sample at a time, with the reference's exact arithmetic, so the
@@ -79,6 +81,7 @@ struct MetricArgs {
     int Pb, ppb;
     int metric;            // 0..13, selects the combiner
     T *r;                  // [Pb][N]
+    const double *rcp;     // [S]: 1/(i+1) in double (Welford running counts)
 };
 
 // visit list entries [off, off+cnt) of PL permutations in order; f(k, s, i)
@@ -134,18 +137,32 @@ __device__ __forceinline__ T combine_stats(int metric, T mA, T vA, T nA,
     return (mA - mB) / sqrt_t(vA / nA + vB / nB);
 }
 
-template <class T, int KIND, int GT, bool XLDS>
+// delta / (i+1), correctly rounded in T.  fp32: (double)delta * fl64(1/(i+1))
+// rounded once to float is the correctly rounded quotient (the product is
+// within 2^-52 of the exact quotient; a quotient of two floats is at least
+// 2^-49 away from every float rounding midpoint), special values included.
+__device__ __forceinline__ float div_count(float delta, int i, const double *rcp) {
+    return (float)((double)delta * rcp[i]);
+}
+__device__ __forceinline__ double div_count(double delta, int i, const double *) {
+    return delta / (double)(i + 1);
+}
+
+template <class T, int KIND, int GT, bool XLDS, int GPL>
 __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
     constexpr int PL = 4;
-    constexpr int PPW = kWave / GT;   // lane groups per wavefront
+    constexpr int LPG = GT / GPL;     // lanes per group
+    constexpr int PPW = kWave / LPG;  // lane groups per wavefront
+    static_assert(GPL == 1 || (XLDS && GPL == 2 && sizeof(T) == 4),
+                  "two genes per lane read one 8-byte pair from the LDS tile");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);
 
     const int g0 = blockIdx.x * GT;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int grp = lane / GT, col = lane % GT;
-    const int g = g0 + col;
+    const int grp = lane / LPG, col = lane % LPG;
+    const int g = g0 + col * GPL;     // first gene of this lane
     const int gc = g < a.N ? g : a.N - 1;
 
     const T *xs;
@@ -157,21 +174,38 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             tile[idx] = gg < a.N ? a.X[(size_t)s * a.N + gg] : T(0);
         }
         __syncthreads();
-        xs = tile + col;
+        xs = tile + col * GPL;
         xstride = GT;
     } else {
         xs = a.X + gc;
         xstride = a.N;
     }
+    // the GPL values of sample s (one 8-byte LDS read for two fp32 genes)
+    auto X_at = [&](int s, T (&v)[GPL]) {
+        if constexpr (GPL == 2) {
+            const float2 p = *reinterpret_cast<const float2 *>(xs + (size_t)s * xstride);
+            v[0] = p.x;
+            v[1] = p.y;
+        } else {
+            v[0] = xs[(size_t)s * xstride];
+        }
+    };
 
     // naive sums see 0*x of the other class (functors.cuh:115-116), which
     // turns into NaN for a non-finite x: track whether the gene has one
-    bool nonfinite = false;
+    bool nonfinite[GPL];
+    bool any_nonfinite = false;
+#pragma unroll
+    for (int j = 0; j < GPL; ++j) nonfinite[j] = false;
     if (KIND == kNaive) {
         for (int s = 0; s < a.S; ++s) {
-            const T v = xs[(size_t)s * xstride];
-            nonfinite |= !(v - v == T(0));
+            T v[GPL];
+            X_at(s, v);
+#pragma unroll
+            for (int j = 0; j < GPL; ++j) nonfinite[j] |= !(v[j] - v[j] == T(0));
         }
+#pragma unroll
+        for (int j = 0; j < GPL; ++j) any_nonfinite |= nonfinite[j];
     }
 
     const T nA = (T)a.nA, nB = (T)a.nB;
@@ -180,7 +214,6 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
     const int p_end = min(a.Pb, p_begin + a.ppb);
     const int slot = wave * PPW + grp;
     const int step = 4 * PPW * PL;
-    auto X_at = [&](int s) -> T { return xs[(size_t)s * xstride]; };
 
     for (int base = p_begin + slot * PL; base < p_end; base += step) {
         const uint16_t *L[PL];
@@ -191,143 +224,225 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             const int pe = pk[k] < p_end ? pk[k] : p_end - 1;
             L[k] = a.lists + (size_t)pe * a.lstride;
         }
-        T out[PL];
+        T out[PL][GPL];
         if (KIND == kNaive || KIND == kOnepass || KIND == kTwopass) {
-            T sA[PL], sB[PL], qA[PL], qB[PL];
+            T sA[PL][GPL], sB[PL][GPL], qA[PL][GPL], qB[PL][GPL];
 #pragma unroll
-            for (int k = 0; k < PL; ++k) sA[k] = sB[k] = qA[k] = qB[k] = T(0);
+            for (int k = 0; k < PL; ++k)
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) sA[k][j] = sB[k][j] = qA[k][j] = qB[k][j] = T(0);
             walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
-                const T v = X_at(s);
-                sA[k] = sA[k] + v;
-                if (KIND == kOnepass) qA[k] = qA[k] + v * v;
+                T v[GPL];
+                X_at(s, v);
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) {
+                    sA[k][j] = sA[k][j] + v[j];
+                    if (KIND == kOnepass) qA[k][j] = qA[k][j] + v[j] * v[j];
+                }
             });
             walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
-                const T v = X_at(s);
-                sB[k] = sB[k] + v;
-                if (KIND == kOnepass) qB[k] = qB[k] + v * v;
-            });
-            if (KIND == kNaive && nonfinite) {
-                bool bA[PL], bB[PL];
+                T v[GPL];
+                X_at(s, v);
 #pragma unroll
-                for (int k = 0; k < PL; ++k) bA[k] = bB[k] = false;
+                for (int j = 0; j < GPL; ++j) {
+                    sB[k][j] = sB[k][j] + v[j];
+                    if (KIND == kOnepass) qB[k][j] = qB[k][j] + v[j] * v[j];
+                }
+            });
+            if (KIND == kNaive && any_nonfinite) {
+                bool bA[PL][GPL], bB[PL][GPL];
+#pragma unroll
+                for (int k = 0; k < PL; ++k)
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) bA[k][j] = bB[k][j] = false;
                 walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
-                    const T v = X_at(s);
-                    bA[k] |= !(v - v == T(0));
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) bA[k][j] |= !(v[j] - v[j] == T(0));
                 });
                 walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
-                    const T v = X_at(s);
-                    bB[k] |= !(v - v == T(0));
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) bB[k][j] |= !(v[j] - v[j] == T(0));
                 });
 #pragma unroll
-                for (int k = 0; k < PL; ++k) {
-                    if (bB[k]) sA[k] = (T)__builtin_nan("");
-                    if (bA[k]) sB[k] = (T)__builtin_nan("");
-                }
-            }
-            T mA[PL], mB[PL];
+                for (int k = 0; k < PL; ++k)
 #pragma unroll
-            for (int k = 0; k < PL; ++k) { mA[k] = sA[k] / nA; mB[k] = sB[k] / nB; }
+                    for (int j = 0; j < GPL; ++j) {
+                        if (bB[k][j]) sA[k][j] = (T)__builtin_nan("");
+                        if (bA[k][j]) sB[k][j] = (T)__builtin_nan("");
+                    }
+            }
+            T mA[PL][GPL], mB[PL][GPL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k)
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) { mA[k][j] = sA[k][j] / nA; mB[k][j] = sB[k][j] / nB; }
             if (KIND == kNaive) {
 #pragma unroll
                 for (int k = 0; k < PL; ++k)
-                    out[k] = combine_means<T>(a.metric, mA[k], mB[k]);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j)
+                        out[k][j] = combine_means<T>(a.metric, mA[k][j], mB[k][j]);
             } else if (KIND == kOnepass) {
                 // functors.cuh:212-218 (biased)
 #pragma unroll
-                for (int k = 0; k < PL; ++k) {
-                    const T vA = qA[k] / nA - mA[k] * mA[k];
-                    const T vB = qB[k] / nB - mB[k] * mB[k];
-                    out[k] = combine_stats<T>(a.metric, mA[k], vA, nA, mB[k], vB, nB);
-                }
+                for (int k = 0; k < PL; ++k)
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T vA = qA[k][j] / nA - mA[k][j] * mA[k][j];
+                        const T vB = qB[k][j] / nB - mB[k][j] * mB[k][j];
+                        out[k][j] = combine_stats<T>(a.metric, mA[k][j], vA, nA, mB[k][j],
+                                                     vB, nB);
+                    }
             } else {
                 // functors.cuh:270-290
                 walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
-                    const T d = X_at(s) - mA[k];
-                    qA[k] = qA[k] + d * d;
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T d = v[j] - mA[k][j];
+                        qA[k][j] = qA[k][j] + d * d;
+                    }
                 });
                 walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
-                    const T d = X_at(s) - mB[k];
-                    qB[k] = qB[k] + d * d;
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T d = v[j] - mB[k][j];
+                        qB[k][j] = qB[k][j] + d * d;
+                    }
                 });
 #pragma unroll
                 for (int k = 0; k < PL; ++k)
-                    out[k] = combine_stats<T>(a.metric, mA[k], qA[k] / nA, nA,
-                                              mB[k], qB[k] / nB, nB);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j)
+                        out[k][j] = combine_stats<T>(a.metric, mA[k][j], qA[k][j] / nA, nA,
+                                                     mB[k][j], qB[k][j] / nB, nB);
             }
         } else if (KIND == kKahan || KIND == kOverkill) {
             // Kahan mean, functors.cuh:144-172 / 316-343
-            T sA[PL], sB[PL], cA[PL], cB[PL];
+            T sA[PL][GPL], sB[PL][GPL], cA[PL][GPL], cB[PL][GPL];
 #pragma unroll
-            for (int k = 0; k < PL; ++k) sA[k] = sB[k] = cA[k] = cB[k] = T(0);
+            for (int k = 0; k < PL; ++k)
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) sA[k][j] = sB[k][j] = cA[k][j] = cB[k][j] = T(0);
             walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
-                const T alpha = X_at(s) - cA[k];
-                const T beta = sA[k] + alpha;
-                cA[k] = (beta - sA[k]) - alpha;
-                sA[k] = beta;
+                T v[GPL];
+                X_at(s, v);
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) {
+                    const T alpha = v[j] - cA[k][j];
+                    const T beta = sA[k][j] + alpha;
+                    cA[k][j] = (beta - sA[k][j]) - alpha;
+                    sA[k][j] = beta;
+                }
             });
             walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
-                const T alpha = X_at(s) - cB[k];
-                const T beta = sB[k] + alpha;
-                cB[k] = (beta - sB[k]) - alpha;
-                sB[k] = beta;
-            });
-            T mA[PL], mB[PL];
+                T v[GPL];
+                X_at(s, v);
 #pragma unroll
-            for (int k = 0; k < PL; ++k) { mA[k] = sA[k] / nA; mB[k] = sB[k] / nB; }
+                for (int j = 0; j < GPL; ++j) {
+                    const T alpha = v[j] - cB[k][j];
+                    const T beta = sB[k][j] + alpha;
+                    cB[k][j] = (beta - sB[k][j]) - alpha;
+                    sB[k][j] = beta;
+                }
+            });
+            T mA[PL][GPL], mB[PL][GPL];
+#pragma unroll
+            for (int k = 0; k < PL; ++k)
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) { mA[k][j] = sA[k][j] / nA; mB[k][j] = sB[k][j] / nB; }
             if (KIND == kKahan) {
 #pragma unroll
                 for (int k = 0; k < PL; ++k)
-                    out[k] = combine_means<T>(a.metric, mA[k], mB[k]);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j)
+                        out[k][j] = combine_means<T>(a.metric, mA[k][j], mB[k][j]);
             } else {
                 // compensated second pass, functors.cuh:346-370
-                T uA[PL], uB[PL], vA[PL], vB[PL];
+                T uA[PL][GPL], uB[PL][GPL], vA[PL][GPL], vB[PL][GPL];
 #pragma unroll
-                for (int k = 0; k < PL; ++k) uA[k] = uB[k] = vA[k] = vB[k] = T(0);
+                for (int k = 0; k < PL; ++k)
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) uA[k][j] = uB[k][j] = vA[k][j] = vB[k][j] = T(0);
                 walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int) {
-                    const T d = X_at(s) - mA[k];
-                    uA[k] = uA[k] + d;
-                    vA[k] = vA[k] + d * d;
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T d = v[j] - mA[k][j];
+                        uA[k][j] = uA[k][j] + d;
+                        vA[k][j] = vA[k][j] + d * d;
+                    }
                 });
                 walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int) {
-                    const T d = X_at(s) - mB[k];
-                    uB[k] = uB[k] + d;
-                    vB[k] = vB[k] + d * d;
+                    T v[GPL];
+                    X_at(s, v);
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T d = v[j] - mB[k][j];
+                        uB[k][j] = uB[k][j] + d;
+                        vB[k][j] = vB[k][j] + d * d;
+                    }
                 });
 #pragma unroll
-                for (int k = 0; k < PL; ++k) {
-                    T va = vA[k] - uA[k] * uA[k] / nA;
-                    T vb = vB[k] - uB[k] * uB[k] / nB;
-                    out[k] = combine_stats<T>(a.metric, mA[k], va / nA, nA,
-                                              mB[k], vb / nB, nB);
-                }
+                for (int k = 0; k < PL; ++k)
+#pragma unroll
+                    for (int j = 0; j < GPL; ++j) {
+                        const T va = vA[k][j] - uA[k][j] * uA[k][j] / nA;
+                        const T vb = vB[k][j] - uB[k][j] * uB[k][j] / nB;
+                        out[k][j] = combine_stats<T>(a.metric, mA[k][j], va / nA, nA,
+                                                     mB[k][j], vb / nB, nB);
+                    }
             }
         } else {
             // Welford/Knuth, functors.cuh:396-428: the i-th sample of a class
             // divides by its running count i+1
-            T mA[PL], mB[PL], vA[PL], vB[PL];
+            T mA[PL][GPL], mB[PL][GPL], vA[PL][GPL], vB[PL][GPL];
 #pragma unroll
-            for (int k = 0; k < PL; ++k) mA[k] = mB[k] = vA[k] = vB[k] = T(0);
+            for (int k = 0; k < PL; ++k)
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) mA[k][j] = mB[k][j] = vA[k][j] = vB[k][j] = T(0);
             walk_lists<PL>(L, 0, a.lenA, [&](int k, int s, int i) {
-                const T v = X_at(s);
-                const T delta = v - mA[k];
-                mA[k] = mA[k] + delta / (T)(i + 1);
-                vA[k] = vA[k] + delta * (v - mA[k]);
+                T v[GPL];
+                X_at(s, v);
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) {
+                    const T delta = v[j] - mA[k][j];
+                    mA[k][j] = mA[k][j] + div_count(delta, i, a.rcp);
+                    vA[k][j] = vA[k][j] + delta * (v[j] - mA[k][j]);
+                }
             });
             walk_lists<PL>(L, a.offB, nRest, [&](int k, int s, int i) {
-                const T v = X_at(s);
-                const T delta = v - mB[k];
-                mB[k] = mB[k] + delta / (T)(i + 1);
-                vB[k] = vB[k] + delta * (v - mB[k]);
+                T v[GPL];
+                X_at(s, v);
+#pragma unroll
+                for (int j = 0; j < GPL; ++j) {
+                    const T delta = v[j] - mB[k][j];
+                    mB[k][j] = mB[k][j] + div_count(delta, i, a.rcp);
+                    vB[k][j] = vB[k][j] + delta * (v[j] - mB[k][j]);
+                }
             });
 #pragma unroll
             for (int k = 0; k < PL; ++k)
-                out[k] = combine_stats<T>(a.metric, mA[k], vA[k] / nA, nA,
-                                          mB[k], vB[k] / nB, nB);
-        }
-        if (g < a.N) {
 #pragma unroll
-            for (int k = 0; k < PL; ++k)
-                if (pk[k] < p_end) a.r[(size_t)pk[k] * a.N + g] = out[k];
+                for (int j = 0; j < GPL; ++j)
+                    out[k][j] = combine_stats<T>(a.metric, mA[k][j], vA[k][j] / nA, nA,
+                                                 mB[k][j], vB[k][j] / nB, nB);
+        }
+#pragma unroll
+        for (int k = 0; k < PL; ++k) {
+            if (pk[k] >= p_end) continue;
+            T *row = a.r + (size_t)pk[k] * a.N;
+#pragma unroll
+            for (int j = 0; j < GPL; ++j)
+                if (g + j < a.N) row[g + j] = out[k][j];
         }
     }
 }
