@@ -15,6 +15,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgsea_hip.so")
 
 SOURCES = ["gsea_engine.hip", "gsea_surrogate.hip", "gsea_proxy.cpp", "gsea_io.cpp"]
+CLI = os.path.join(HERE, "cudagsea")   # standalone command line (gsea_cli.cpp)
 
 # IEEE arithmetic in source order: no contraction, no fast-math; packed FP32
 # instructions disabled for the device code.
@@ -51,7 +52,25 @@ def build(verbose=False):
     if verbose and out.strip():
         print(out)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose=False):
+    """The command line links the library (host code only, g++)."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = CLI + ".tmp"
+    cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", CSRC,
+           os.path.join(CSRC, "gsea_cli.cpp"), "-L", HERE, "-lgsea_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                          text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("g++ failed for the command line:\n" + proc.stdout)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

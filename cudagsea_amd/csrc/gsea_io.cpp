@@ -3,8 +3,8 @@
 // cudaGSEA/src/include/read_data_files.cuh:46-321 (header checks, symbol
 // trimming, duplicate-symbol rejection, symbolic or 0/1 class labels); where
 // the reference calls exit() with an error code these functions throw
-// std::runtime_error carrying the same code, which Rcpp turns into an R error
-// instead of ending the R session.
+// gsea::Error (a std::runtime_error) carrying the same code, which Rcpp turns
+// into an R error instead of ending the R session.
 #include <algorithm>
 #include <cctype>
 #include <fstream>
@@ -15,14 +15,12 @@
 #include <vector>
 
 #include "cudaGSEA_wrapper.cuh"
+#include "gsea_error.hpp"
 #include "gsea_hip.h"
 
 namespace {
 
-struct IoError : std::runtime_error {
-    int code;
-    IoError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
+using IoError = gsea::Error;
 
 std::string trim(std::string s) {
     auto sp = [](unsigned char c) { return std::isspace(c) != 0; };

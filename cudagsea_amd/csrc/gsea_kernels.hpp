@@ -715,6 +715,21 @@ __device__ __forceinline__ void es_mark_set(const int32_t *sg, const uint16_t *p
 // fmin / fmax skip it, as the reference's comparison skips a NaN running
 // sum).  EXACT == true: L starts at the lane's Sa and every candidate goes
 // through es_candidates (first strict maximum, rounded to T).
+// min / max of two doubles with NaN operands skipped (IEEE-mode v_min_f64 /
+// v_max_f64).  The walk's accumulators only ever hold quiet NaN or results
+// of these instructions, so the canonicalisation the compiler adds around
+// fmin / fmax (for signalling NaN inputs) is not needed.
+__device__ __forceinline__ double min_skip_nan(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double max_skip_nan(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <class T, int W, bool EXACT, class AF>
 __device__ __forceinline__ void es_walk(uint32_t *bm, AF &A,
                                         const HitShare<T> &share, int lane,
@@ -722,25 +737,26 @@ __device__ __forceinline__ void es_walk(uint32_t *bm, AF &A,
                                         double &L, double &vmin, double &vmax,
                                         T &best) {
     uint32_t cur = 0;
-    int base = 0;
-    for (int t = 0; t < hits; ++t) {
+    int wq = 0;        // first rank of the current word
+    int jt = j;        // set index of the current hit
+    for (int t = 0; t < hits; ++t, ++jt) {
         if (cur == 0u) {
-            const int i = __ffs(nz) - 1;
+            const int w = lane * W + __builtin_ctz(nz);
             nz &= nz - 1u;
-            cur = bm[lane * W + i];
-            bm[lane * W + i] = 0u;   // leaves the bitmap clear for the next set
-            base = (lane * W + i) * 32;
+            cur = bm[w];
+            bm[w] = 0u;   // leaves the bitmap clear for the next set
+            wq = w * 32;
         }
-        const int q = base + __ffs(cur) - 1;
+        const int q = wq + __builtin_ctz(cur);
         cur &= cur - 1u;
         const double c = (double)share(A(q));
         if (EXACT) {
-            es_candidates<T>(L, c, q, j + t, dec, best);
+            es_candidates<T>(L, c, q, jt, dec, best);
         } else {
-            const double md = (double)(q - j - t) * dec;
-            vmin = fmin(vmin, L + md);
+            const double md = (double)(q - jt) * dec;
+            vmin = min_skip_nan(vmin, L + md);
             L += c;
-            vmax = fmax(vmax, L + md);
+            vmax = max_skip_nan(vmax, L + md);
         }
     }
 }
