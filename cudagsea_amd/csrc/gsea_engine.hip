@@ -66,7 +66,7 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kNumBufs
+    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kBufGenes16, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -404,12 +404,13 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
 }
 
 // Work split of the ES phase: sets above kSmallSet genes are scored by one
-// wavefront each (largest first), the others by one lane each in chunks of
+// wavefront each (in set order), the others by one lane each in chunks of
 // 64 sets of one size class (<= 8, <= 16, <= 32 genes).
 struct SetTiers {
     int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, stride = 0;
     const int4 *large_desc = nullptr;
     const int32_t *small_ids = nullptr, *small_k = nullptr, *small_gm = nullptr;
+    const uint16_t *genes16 = nullptr;   // 16-bit copy of the CSR gene ids
     const void *decay = nullptr;         // [G] of T
     const void *large_decay = nullptr;   // [n_large] of T
 };
@@ -424,6 +425,20 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     HIP_TRY(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (G + 1),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    // 16-bit gene ids for k_es (the caller's CSR may change between calls at
+    // the same address, so the copy is made on every call; ~13 MB at c3)
+    {
+        const int64_t total = off[(size_t)G];
+        void *p16;
+        int r = ensure(d, kBufGenes16, sizeof(uint16_t) * std::max<int64_t>(total, 1), &p16);
+        if (r) return r;
+        if (total > 0) {
+            hipLaunchKernelGGL(k_genes16, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                               0, st, d_genes, total, (uint16_t *)p16);
+            HIP_TRY(hipGetLastError());
+        }
+        t->genes16 = (const uint16_t *)p16;
+    }
     std::vector<int32_t> all;
     if (!subset) {
         all.resize((size_t)G);
@@ -437,9 +452,10 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
         if (k > kSmallSet) large.push_back((int32_t)s);
         else cls[k <= 8 ? 0 : k <= 16 ? 1 : 2].push_back((int32_t)s);
     }
-    std::stable_sort(large.begin(), large.end(), [&](int32_t x, int32_t y) {
-        return off[x + 1] - off[x] > off[y + 1] - off[y];
-    });
+    // large sets stay in set order: a workgroup's wavefronts claim items in
+    // order, so the ES stores of one permutation row fill cache lines
+    // together (with thousands of items per permutation the tail imbalance
+    // of this order is a few microseconds)
     std::vector<int32_t> sid, sk;
     int ends[3];
     for (int c = 0; c < 3; ++c) {
@@ -515,6 +531,7 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.n_items = tiers ? tiers->n_large + tiers->n_chunks : 0;
     a.large_desc = tiers ? tiers->large_desc : nullptr;
     a.large_decay = tiers ? (const T *)tiers->large_decay : nullptr;
+    a.set_genes16 = tiers ? tiers->genes16 : nullptr;
     a.small_ids = tiers ? tiers->small_ids : nullptr;
     a.small_k = tiers ? tiers->small_k : nullptr;
     a.small_gm = tiers ? tiers->small_gm : nullptr;

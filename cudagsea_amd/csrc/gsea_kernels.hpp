@@ -483,6 +483,7 @@ struct RankArgs {
     int G;
     const int32_t *set_off;  // [G+1]
     const int32_t *set_genes;
+    const uint16_t *set_genes16;  // the same CSR entries as 16-bit gene ids (k_es)
     T *es;                   // [P][G]
     T *a_glob;               // [Pb][N] |metric| by rank when it is not in LDS
     uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
@@ -495,7 +496,7 @@ struct RankArgs {
     // chunks of 64 small sets (one lane each) grouped by size class
     int n_items, n_large;
     const int4 *large_desc;            // [n_large] {set id, offset, size, 0},
-                                       // largest first
+                                       // in set order
     const T *large_decay;              // [n_large] decay of those sets
     const int32_t *small_ids;          // [n_chunks*64] set id or -1
     const int32_t *small_k;            // [n_chunks*64] set sizes
@@ -506,6 +507,14 @@ struct RankArgs {
 };
 
 constexpr int kSmallSet = 32;
+
+// 16-bit copy of the set CSR's gene ids (gene counts are below 2^15): half
+// the bytes every permutation's workgroup streams through in k_es
+__global__ void k_genes16(const int32_t *__restrict__ genes, int64_t n,
+                          uint16_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint16_t)genes[i];
+}
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
 // read row t of 64 consecutive slots
@@ -673,12 +682,12 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
 // their |r| summed.  Slots past the set's end repeat its last gene (genes of
 // a set are distinct; OR-ing its bit again changes nothing) and add 0.
 template <class T, int B, class AF>
-__device__ __forceinline__ void es_mark_hits(const int32_t *sg, const uint16_t *pos,
+__device__ __forceinline__ void es_mark_hits(const uint16_t *sg, const uint16_t *pos,
                                              AF &A, uint32_t *bm, int k, int j0,
                                              int lane, double &nr) {
     int q[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u) q[u] = sg[min(j0 + u * kWave + lane, k - 1)];
+    for (int u = 0; u < B; ++u) q[u] = (int)sg[min(j0 + u * kWave + lane, k - 1)];
 #pragma unroll
     for (int u = 0; u < B; ++u) q[u] = pos[q[u]];
     T x[B];
@@ -695,7 +704,7 @@ __device__ __forceinline__ void es_mark_hits(const int32_t *sg, const uint16_t *
 // All of a set's entries through es_mark_hits, 8 slots per lane per round;
 // the last round takes the smallest batch that covers the set's end.
 template <class T, class AF>
-__device__ __forceinline__ void es_mark_set(const int32_t *sg, const uint16_t *pos,
+__device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *pos,
                                             AF &A, uint32_t *bm, int k, int lane,
                                             double &nr) {
     for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
@@ -776,7 +785,7 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
                                              int o0, int k, double decay, int lane) {
-    const int32_t *sg = a.set_genes + o0;
+    const uint16_t *sg = a.set_genes16 + o0;
     double nr = 0.0;
     es_mark_set<T>(sg, pos, A, bm, k, lane, nr);
     nr = wave_sum(nr);
