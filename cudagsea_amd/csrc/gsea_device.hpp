@@ -139,6 +139,10 @@ __device__ __forceinline__ float key_abs(uint32_t k, bool asc) {
     return __uint_as_float(u & 0x7fffffffu);
 }
 
+// dword 3 of a raw buffer resource on gfx9-family devices (32-bit data
+// format, no swizzle, bounds checked against num_records bytes)
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
 // ---------------------------------------------------------------------------
 // wavefront primitives
 // ---------------------------------------------------------------------------

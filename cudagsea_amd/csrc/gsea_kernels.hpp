@@ -685,20 +685,26 @@ template <class T, int B, class AF>
 __device__ __forceinline__ void es_mark_hits(const uint16_t *sg, const uint16_t *pos,
                                              AF &A, uint32_t *bm, int k, int j0,
                                              int lane, double &nr) {
-    int q[B];
+    // buffer loads: the set's entries as one resource (scalar base, bounds
+    // k entries), a 32-bit byte offset per slot and no 64-bit address math
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)sg, (short)0, 2 * k, kBufferRsrcWord3);
+    uint32_t q[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u) q[u] = (int)sg[min(j0 + u * kWave + lane, k - 1)];
+    for (int u = 0; u < B; ++u)
+        q[u] = __builtin_amdgcn_raw_buffer_load_b16(
+            rs, 2 * min(j0 + u * kWave + lane, k - 1), 0, 0);
 #pragma unroll
     for (int u = 0; u < B; ++u) q[u] = pos[q[u]];
     T x[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-        x[u] = A(q[u]);
-        atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31));
+        x[u] = A((int)q[u]);
+        atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31u));
     }
 #pragma unroll
     for (int u = 0; u < B; ++u)
-        nr += j0 + u * kWave + lane < k ? (double)x[u] : 0.0;
+        nr += (double)(j0 + u * kWave + lane < k ? x[u] : T(0));
 }
 
 // All of a set's entries through es_mark_hits, 8 slots per lane per round;
@@ -1015,16 +1021,21 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
         if (w < a.es_waves) {
             uint32_t *bm = bms + (size_t)w * a.nw;
             const int64_t gp = a.perm_base + a.p_lo + p;
+            // work-item indices and descriptors are wavefront-uniform: they
+            // live in scalar registers (scalar loads, scalar address math)
             int item = 0;
             if (lane == 0) item = atomicAdd(&misc[0], 1);
-            item = __shfl(item, 0, kWave);
+            item = __builtin_amdgcn_readfirstlane(item);
             while (item < a.n_items) {
                 // fetch the next work item now; its latency hides behind this one
                 int next = 0;
                 if (lane == 0) next = atomicAdd(&misc[0], 1);
                 if (item < a.n_large) {
                     const int4 d = a.large_desc[item];
-                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp, d.x, d.y, d.z,
+                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp,
+                                             __builtin_amdgcn_readfirstlane(d.x),
+                                             __builtin_amdgcn_readfirstlane(d.y),
+                                             __builtin_amdgcn_readfirstlane(d.z),
                                              (double)a.large_decay[item], lane);
                 } else {
                     const int chunk = item - a.n_large;
@@ -1035,7 +1046,7 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                     else
                         es_small_sets<T, 32>(a, pos, A, gp, chunk, lane);
                 }
-                item = __shfl(next, 0, kWave);
+                item = __builtin_amdgcn_readfirstlane(next);
             }
         }
         __syncthreads();
