@@ -66,7 +66,8 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kBufGenes16, kNumBufs
+    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kBufGenes16,
+    kBufExtSorted, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -624,21 +625,45 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
     if (rc) return rc;
     T *factor = (T *)pf, *maxv = (T *)pmax, *minv = (T *)pmin;
     const int g = (int)G;
-    hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 255) / 256), dim3(256), 0, st,
+    // one wavefront per 64 sets: the sequential per-set sums are latency
+    // bound, so spread them over as many CUs as possible
+    hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 63) / 64), dim3(64), 0, st,
                        es_pg, P, g, d_stats, factor, d_budget, d_cv);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_perm_extrema<T>, dim3((unsigned)((P + 3) / 4)), dim3(256), 0,
                        st, es_pg, P, g, factor, d_budget, maxv, minv);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
-                       P, g, maxv, minv, d_budget);
-    HIP_TRY(hipGetLastError());
+    // host buffers of this function feed asynchronous copies: the function
+    // synchronises the stream before they go out of scope
+    std::vector<T> nes(G), ext;
+    if (d_budget) {
+        hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
+                           P, g, maxv, minv, d_budget);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        // FWER by binary search in the sorted per-permutation extrema
+        ext.resize((size_t)2 * P);
+        HIP_TRY(hipMemcpyAsync(ext.data(), maxv, sizeof(T) * P, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(ext.data() + P, minv, sizeof(T) * P, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        std::sort(ext.begin(), ext.begin() + P);
+        std::sort(ext.begin() + P, ext.end());
+        void *psorted;
+        rc = ensure(d, kBufExtSorted, sizeof(T) * 2 * P, &psorted);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(psorted, ext.data(), sizeof(T) * 2 * P,
+                               hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fwer_sorted<T>, dim3((g + 255) / 256), dim3(256), 0, st,
+                           d_stats, P, g, (const T *)psorted, (const T *)psorted + P);
+        HIP_TRY(hipGetLastError());
+    }
 
     // FDR: thresholds are the sorted non-NaN NES (sorted on the host: G values)
-    std::vector<T> nes(G);
-    HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
     std::vector<T> thr;
     thr.reserve(G);
     for (T v : nes)
@@ -677,6 +702,7 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
     hipLaunchKernelGGL(k_fdr_final<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
                        g, (const T *)pthr, n, hub, hlb, signs);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
     return GSEA_OK;
 }
 

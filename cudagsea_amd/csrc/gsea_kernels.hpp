@@ -1076,16 +1076,17 @@ __global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
     double s1 = 0.0, s2 = 0.0;
     int64_t c = 0;
     int64_t p = 0;
-    for (; p + 4 <= B; p += 4) {
-        const T v0 = __builtin_fabs(es[(p + 0) * G + s]);
-        const T v1 = __builtin_fabs(es[(p + 1) * G + s]);
-        const T v2 = __builtin_fabs(es[(p + 2) * G + s]);
-        const T v3 = __builtin_fabs(es[(p + 3) * G + s]);
-        sum += v0; sum += v1; sum += v2; sum += v3;
-        c += (v0 >= ref) + (v1 >= ref) + (v2 >= ref) + (v3 >= ref);
-        if (cv) {
-            s1 += (double)v0 + (double)v1 + (double)v2 + (double)v3;
-            s2 += (double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2 + (double)v3 * v3;
+    // 32 loads in flight per lane; the sum stays sequential in p
+    constexpr int U = 32;
+    for (; p + U <= B; p += U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_fabs(es[(p + u) * G + s]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            sum += v[u];
+            c += v[u] >= ref;
+            if (cv) { s1 += (double)v[u]; s2 += (double)v[u] * v[u]; }
         }
     }
     for (; p < B; ++p) {
@@ -1150,6 +1151,42 @@ __global__ void k_fwer(T *__restrict__ out, int64_t P, int G,
         for (int64_t p = 0; p < B; ++p) c += minv[p] <= score;
     }
     out[3 * G + s] = (T)c / (T)B;
+}
+
+// FWER from the per-permutation extrema sorted ascending (same counts as
+// k_fwer's loop: #max >= NES for NES > 0, #min <= NES otherwise; NaN NES
+// counts nothing)
+template <class T>
+__device__ __forceinline__ int64_t count_ge(const T *v, int64_t n, T x) {
+    int64_t lo = 0, hi = n;   // first index with v >= x
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (v[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return n - lo;
+}
+
+template <class T>
+__device__ __forceinline__ int64_t count_le(const T *v, int64_t n, T x) {
+    int64_t lo = 0, hi = n;   // first index with v > x
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (v[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__global__ void k_fwer_sorted(T *__restrict__ out, int64_t P, int G,
+                              const T *__restrict__ max_sorted,
+                              const T *__restrict__ min_sorted) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G) return;
+    const T score = out[G + s];
+    int64_t c = 0;
+    if (score > T(0)) c = count_ge(max_sorted, P, score);
+    else if (score == score) c = count_le(min_sorted, P, score);
+    out[3 * G + s] = (T)c / (T)P;
 }
 
 template <class T>
