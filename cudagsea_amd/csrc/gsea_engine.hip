@@ -228,8 +228,10 @@ static hipError_t launch_metric_gt(const MetricArgs<T> &a, bool xlds,
                                    hipStream_t st) {
     const int gx = (a.N + GT - 1) / GT;
     const int gy = (a.Pb + a.ppb - 1) / a.ppb;
-    // fp32 with the X tile in LDS: two genes per lane
-    constexpr int GPL = sizeof(T) == 4 ? 2 : 1;
+    // fp32 with the X tile in LDS: four genes per lane (two for the
+    // overkill accumulator, whose eight running values per gene and
+    // permutation would not fit in registers at four)
+    constexpr int GPL = sizeof(T) != 4 ? 1 : (KIND == kOverkill ? 2 : 4);
     if (xlds) {
         const size_t lds = (size_t)a.S * GT * sizeof(T);
         hipLaunchKernelGGL((k_metric<T, KIND, GT, true, GPL>), dim3(gx, gy), dim3(256),
