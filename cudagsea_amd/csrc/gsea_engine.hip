@@ -228,10 +228,8 @@ static hipError_t launch_metric_gt(const MetricArgs<T> &a, bool xlds,
                                    hipStream_t st) {
     const int gx = (a.N + GT - 1) / GT;
     const int gy = (a.Pb + a.ppb - 1) / a.ppb;
-    // fp32 with the X tile in LDS: four genes per lane (two for the
-    // overkill accumulator, whose eight running values per gene and
-    // permutation would not fit in registers at four)
-    constexpr int GPL = sizeof(T) != 4 ? 1 : (KIND == kOverkill ? 2 : 4);
+    // X tile in LDS: one 16-byte read per sample, 4 fp32 or 2 fp64 genes
+    constexpr int GPL = sizeof(T) == 4 ? 4 : 2;
     if (xlds) {
         const size_t lds = (size_t)a.S * GT * sizeof(T);
         hipLaunchKernelGGL((k_metric<T, KIND, GT, true, GPL>), dim3(gx, gy), dim3(256),

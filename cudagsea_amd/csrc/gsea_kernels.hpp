@@ -55,9 +55,9 @@ __global__ void k_class_lists(const uint8_t *__restrict__ labels, int S,
 // Block = 4 wavefronts.  The block stages X[0:S][g0:g0+GT] (sample-major
 // input) in LDS once and streams `ppb` permutations over it.  A wavefront is
 // split into lane groups of GT/GPL lanes; lane `col` of a group owns genes
-// g0+col*GPL .. +GPL-1 (fp32: GPL = 4, or 2 for the overkill accumulator
-// whose state would not fit; one 16/8-byte LDS read per sample serves them
-// all, so the class-list decoding and addressing is shared), and
+// g0+col*GPL .. +GPL-1 (GPL = 4 fp32 or 2 fp64 genes: one 16-byte LDS read
+// per sample serves them all, so the class-list decoding and addressing is
+// shared), and
 // each lane carries PL permutations at once (independent accumulation
 // chains).  Every class sum walks that class's samples in increasing sample
 // order, one sample at a time, with the reference's exact arithmetic, so the
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
     constexpr int PL = 4;
     constexpr int LPG = GT / GPL;     // lanes per group
     constexpr int PPW = kWave / LPG;  // lane groups per wavefront
-    static_assert(GPL == 1 || (XLDS && (GPL == 2 || GPL == 4) && sizeof(T) == 4),
+    static_assert(GPL == 1 || (XLDS && (GPL == 2 || GPL == 4) && sizeof(T) * GPL <= 16),
                   "several genes per lane read one 8/16-byte group from the LDS tile");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *tile = reinterpret_cast<T *>(smem_raw);
@@ -190,6 +190,10 @@ __global__ __launch_bounds__(256) void k_metric(MetricArgs<T> a) {
             v[1] = p.y;
             v[2] = p.z;
             v[3] = p.w;
+        } else if constexpr (GPL == 2 && sizeof(T) == 8) {
+            const double2 p = *reinterpret_cast<const double2 *>(xs + (size_t)s * xstride);
+            v[0] = p.x;
+            v[1] = p.y;
         } else if constexpr (GPL == 2) {
             const float2 p = *reinterpret_cast<const float2 *>(xs + (size_t)s * xstride);
             v[0] = p.x;
