@@ -67,7 +67,7 @@ enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
     kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kBufGenes16,
-    kBufExtSorted, kNumBufs
+    kBufExtSorted, kBufFdrTable, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -636,6 +636,7 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
     // host buffers of this function feed asynchronous copies: the function
     // synchronises the stream before they go out of scope
     std::vector<T> nes(G), ext;
+    std::vector<int32_t> table;
     if (d_budget) {
         hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
                            P, g, maxv, minv, d_budget);
@@ -680,20 +681,46 @@ static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
     if (n) HIP_TRY(hipMemcpyAsync(pthr, thr.data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(phist, 0, sizeof(unsigned long long) * (2 * (n + 1) + 2), st));
     if (P > 1) {
-        const size_t lds = align16(sizeof(T) * n) + 2 * sizeof(uint32_t) * (n + 1);
-        const int64_t total = (P - 1) * G;
-        int blocks = (int)std::min<int64_t>((total + 255) / 256, 4LL * d->cu_count);
+        // bucket table for the threshold search: table[b] = #thresholds below
+        // lo + b*w, nb uniform buckets over [thr[0], thr[n-1]]
+        int nb = 0;
+        T lo = T(0), inv_w = T(0);
+        if (n > 1 && thr[n - 1] > thr[0]) {
+            nb = (int)std::min<int64_t>(4096, n);
+            lo = thr[0];
+            const double w = ((double)thr[n - 1] - (double)thr[0]) / nb;
+            inv_w = (T)(1.0 / w);
+            table.resize((size_t)nb + 1);
+            for (int b = 0; b <= nb; ++b) {
+                const double edge = (double)thr[0] + b * w;
+                table[(size_t)b] = (int32_t)(std::lower_bound(thr.begin(), thr.end(), edge,
+                                                              [](T x, double e) {
+                                                                  return (double)x < e;
+                                                              }) - thr.begin());
+            }
+            table[0] = 0;                  // bounds, whatever the edges' rounding
+            table[(size_t)nb] = n;
+        }
+        void *ptab;
+        rc = ensure(d, kBufFdrTable, sizeof(int32_t) * (size_t)(nb + 1), &ptab);
+        if (rc) return rc;
+        if (nb)
+            HIP_TRY(hipMemcpyAsync(ptab, table.data(), sizeof(int32_t) * (nb + 1),
+                                   hipMemcpyHostToDevice, st));
+        const size_t lds = align16(sizeof(T) * n) + align16(sizeof(int32_t) * (nb + 1)) +
+                           2 * sizeof(uint32_t) * (n + 1);
+        const int blocks = (int)std::min<int64_t>(P - 1, 4LL * d->cu_count);
         if (lds <= (size_t)d->lds_max) {
             HIP_TRY(hipFuncSetAttribute((const void *)k_fdr_hist<T, true>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds));
             hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), lds, st,
-                               es_pg, P, g, factor, d_budget, (const T *)pthr, n, hub, hlb,
-                               signs);
+                               es_pg, P, g, factor, d_budget, (const T *)pthr, n,
+                               (const int32_t *)ptab, nb, lo, inv_w, hub, hlb, signs);
         } else {
             hipLaunchKernelGGL((k_fdr_hist<T, false>), dim3(blocks), dim3(256), 0, st,
-                               es_pg, P, g, factor, d_budget, (const T *)pthr, n, hub, hlb,
-                               signs);
+                               es_pg, P, g, factor, d_budget, (const T *)pthr, n,
+                               (const int32_t *)ptab, 0, lo, inv_w, hub, hlb, signs);
         }
         HIP_TRY(hipGetLastError());
     }

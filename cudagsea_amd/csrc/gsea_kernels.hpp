@@ -1231,48 +1231,74 @@ __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
                            const T *__restrict__ factor,
                            const int32_t *__restrict__ budget,
                            const T *__restrict__ thr, int n,
+                           const int32_t *__restrict__ table, int nb, T lo, T inv_w,
                            unsigned long long *__restrict__ hub,
                            unsigned long long *__restrict__ hlb,
                            unsigned long long *__restrict__ signs) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const T *t = thr;
     uint32_t *h1 = nullptr, *h2 = nullptr;
+    int32_t *tb = nullptr;
     if (LDS) {
         T *tl = reinterpret_cast<T *>(smem_raw);
-        h1 = reinterpret_cast<uint32_t *>(smem_raw + ((sizeof(T) * n + 15) / 16) * 16);
+        tb = reinterpret_cast<int32_t *>(smem_raw + ((sizeof(T) * n + 15) / 16) * 16);
+        h1 = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(tb) +
+                                          ((sizeof(int32_t) * (nb + 1) + 15) / 16) * 16);
         h2 = h1 + (n + 1);
         for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = thr[i];
+        for (int i = threadIdx.x; i <= nb; i += blockDim.x) tb[i] = table[i];
         for (int i = threadIdx.x; i <= n; i += blockDim.x) { h1[i] = 0; h2[i] = 0; }
         __syncthreads();
         t = tl;
     }
     unsigned long long npos = 0, nneg = 0;
-    const int64_t total = (P - 1) * (int64_t)G;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = 1 + e / G;
-        const int s = (int)(e % G);
-        // a set scored for B of the P permutations stands for P/B null
-        // samples per score (budgets divide P)
-        uint32_t w = 1;
-        if (budget) {
-            if (p >= budget[s]) continue;
-            w = (uint32_t)(P / budget[s]);
+    // whole permutation rows per block (no index division per score)
+    for (int64_t p = 1 + blockIdx.x; p < P; p += gridDim.x) {
+        for (int s = threadIdx.x; s < G; s += blockDim.x) {
+            // a set scored for B of the P permutations stands for P/B null
+            // samples per score (budgets divide P)
+            uint32_t w = 1;
+            if (budget) {
+                if (p >= budget[s]) continue;
+                w = (uint32_t)(P / budget[s]);
+            }
+            const T z = es[p * G + s] * factor[s];
+            if (z != z) continue;
+            int lb;
+            if (LDS && nb > 0) {
+                // uniform buckets over [t[0], t[n-1]]: the bucket's table
+                // entries (one bucket of slack each side for the rounding of
+                // the bucket index) bound the binary search
+                if (!(z > t[0])) {
+                    lb = 0;
+                } else if (z > t[n - 1]) {
+                    lb = n;
+                } else {
+                    int b = (int)((z - lo) * inv_w);
+                    b = b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+                    int a = tb[b > 0 ? b - 1 : 0];
+                    int c = tb[b + 2 <= nb ? b + 2 : nb];
+                    while (a < c) {
+                        const int mid = (a + c) >> 1;
+                        if (t[mid] < z) a = mid + 1; else c = mid;
+                    }
+                    lb = a;
+                }
+            } else {
+                lb = lower_bound_t(t, n, z);
+            }
+            int ub = lb;                       // thresholds equal to z are rare
+            while (ub < n && !(z < t[ub])) ++ub;
+            if (LDS) {
+                atomicAdd(&h1[ub], w);
+                atomicAdd(&h2[lb], w);
+            } else {
+                atomicAdd(&hub[ub], (unsigned long long)w);
+                atomicAdd(&hlb[lb], (unsigned long long)w);
+            }
+            npos += z >= T(0) ? w : 0u;
+            nneg += z <= T(0) ? w : 0u;
         }
-        const T z = es[p * G + s] * factor[s];
-        if (z != z) continue;
-        const int lb = lower_bound_t(t, n, z);
-        int ub = lb;                       // thresholds equal to z are rare
-        while (ub < n && !(z < t[ub])) ++ub;
-        if (LDS) {
-            atomicAdd(&h1[ub], w);
-            atomicAdd(&h2[lb], w);
-        } else {
-            atomicAdd(&hub[ub], (unsigned long long)w);
-            atomicAdd(&hlb[lb], (unsigned long long)w);
-        }
-        npos += z >= T(0) ? w : 0u;
-        nneg += z <= T(0) ? w : 0u;
     }
     npos = wave_sum(npos);
     nneg = wave_sum(nneg);
