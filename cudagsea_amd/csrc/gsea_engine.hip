@@ -406,11 +406,14 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
 
 // Work split of the ES phase: sets above kSmallSet genes are scored by one
 // wavefront each (in set order), the others by one lane each in chunks of
-// 64 sets of one size class (<= 8, <= 16, <= 32 genes).
+// 64 sets of one size class (<= 8, <= 16, <= 32, <= 64, <= 128 genes).
 struct SetTiers {
-    int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, stride = 0;
+    int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, chunk_end32 = 0;
+    int chunk_end64 = 0;
+    int stride = 0;
     const int4 *large_desc = nullptr;
-    const int32_t *small_ids = nullptr, *small_k = nullptr, *small_gm = nullptr;
+    const int32_t *small_ids = nullptr, *small_k = nullptr;
+    const uint16_t *small_gm = nullptr;
     const uint16_t *genes16 = nullptr;   // 16-bit copy of the CSR gene ids
     const void *decay = nullptr;         // [G] of T
     const void *large_decay = nullptr;   // [n_large] of T
@@ -446,28 +449,30 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
         for (int64_t s = 0; s < G; ++s) all[(size_t)s] = (int32_t)s;
         subset = &all;
     }
-    std::vector<int32_t> large, cls[3];
+    std::vector<int32_t> large, cls[5];
     for (int32_t s : *subset) {
         const int k = off[(size_t)s + 1] - off[(size_t)s];
         if (k < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
         if (k > kSmallSet) large.push_back((int32_t)s);
-        else cls[k <= 8 ? 0 : k <= 16 ? 1 : 2].push_back((int32_t)s);
+        else cls[k <= 8 ? 0 : k <= 16 ? 1 : k <= 32 ? 2 : k <= 64 ? 3 : 4].push_back((int32_t)s);
     }
     // large sets stay in set order: a workgroup's wavefronts claim items in
     // order, so the ES stores of one permutation row fill cache lines
     // together (with thousands of items per permutation the tail imbalance
     // of this order is a few microseconds)
     std::vector<int32_t> sid, sk;
-    int ends[3];
-    for (int c = 0; c < 3; ++c) {
+    int ends[5];
+    for (int c = 0; c < 5; ++c) {
         for (int32_t s : cls[c]) { sid.push_back(s); sk.push_back(off[s + 1] - off[s]); }
         while (sid.size() % kWave) { sid.push_back(-1); sk.push_back(0); }
         ends[c] = (int)(sid.size() / kWave);
     }
     t->n_large = (int)large.size();
-    t->n_chunks = ends[2];
+    t->n_chunks = ends[4];
     t->chunk_end8 = ends[0];
     t->chunk_end16 = ends[1];
+    t->chunk_end32 = ends[2];
+    t->chunk_end64 = ends[3];
     t->stride = (int)sid.size();
     const size_t nl = large.size(), ns = sid.size();
     // int32 words: large descriptors {set, offset, size, 0} | small ids | small sizes
@@ -491,7 +496,7 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     Buffer *tb = d->tier_bufs[slot];
     int rc = ensure_buf(tb[kTierIds], sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
     if (!rc) rc = ensure_buf(tb[kTierDecay], sizeof(T) * std::max<size_t>(decay.size(), 1), &pd);
-    if (!rc) rc = ensure_buf(tb[kTierGm], sizeof(int32_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
+    if (!rc) rc = ensure_buf(tb[kTierGm], sizeof(uint16_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
     if (rc) return rc;
     if (!host.empty())
         HIP_TRY(hipMemcpyAsync(pt, host.data(), sizeof(int32_t) * host.size(),
@@ -504,10 +509,10 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     t->large_desc = (const int4 *)pt;
     t->small_ids = (const int32_t *)pt + 4 * nl;
     t->small_k = (const int32_t *)pt + 4 * nl + ns;
-    t->small_gm = (const int32_t *)pg;
+    t->small_gm = (const uint16_t *)pg;
     if (ns) {
         hipLaunchKernelGGL(k_small_gm, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
-                           d_off, d_genes, t->small_ids, (int)ns, (int32_t *)pg);
+                           d_off, d_genes, t->small_ids, (int)ns, (uint16_t *)pg);
         HIP_TRY(hipGetLastError());
     }
     // the host vectors above must outlive the asynchronous copy
@@ -539,6 +544,8 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.small_stride = tiers ? tiers->stride : 0;
     a.chunk_end8 = tiers ? tiers->chunk_end8 : 0;
     a.chunk_end16 = tiers ? tiers->chunk_end16 : 0;
+    a.chunk_end32 = tiers ? tiers->chunk_end32 : 0;
+    a.chunk_end64 = tiers ? tiers->chunk_end64 : 0;
     a.decay = tiers ? (const T *)tiers->decay : nullptr;
 }
 

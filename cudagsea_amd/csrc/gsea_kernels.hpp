@@ -512,13 +512,13 @@ struct RankArgs {
     const T *large_decay;              // [n_large] decay of those sets
     const int32_t *small_ids;          // [n_chunks*64] set id or -1
     const int32_t *small_k;            // [n_chunks*64] set sizes
-    const int32_t *small_gm;           // [32][small_stride] gene ids
+    const uint16_t *small_gm;          // [kSmallSet][small_stride] gene ids
     int small_stride;
-    int chunk_end8, chunk_end16;       // chunk class boundaries
+    int chunk_end8, chunk_end16, chunk_end32, chunk_end64;   // chunk class boundaries
     const T *decay;                    // [G] (T)(-1.0 / (double)(N - k_s))
 };
 
-constexpr int kSmallSet = 32;
+constexpr int kSmallSet = 128;
 
 // 16-bit copy of the set CSR's gene ids (gene counts are below 2^15): half
 // the bytes every permutation's workgroup streams through in k_es
@@ -533,14 +533,14 @@ __global__ void k_genes16(const int32_t *__restrict__ genes, int64_t n,
 __global__ void k_small_gm(const int32_t *__restrict__ off,
                            const int32_t *__restrict__ genes,
                            const int32_t *__restrict__ small_ids, int nslots,
-                           int32_t *__restrict__ gm) {
+                           uint16_t *__restrict__ gm) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= nslots) return;
     const int s = small_ids[slot];
     const int o0 = s >= 0 ? off[s] : 0;
     const int k = s >= 0 ? off[s + 1] - o0 : 0;
     for (int t = 0; t < kSmallSet; ++t)
-        gm[(size_t)t * nslots + slot] = t < k ? genes[o0 + t] : 0;
+        gm[(size_t)t * nslots + slot] = (uint16_t)(t < k ? genes[o0 + t] : 0);
 }
 
 template <class T>
@@ -607,69 +607,132 @@ __device__ __forceinline__ T wave_first_absmax(T x) {
     return read_lane(x, kWave - 1);
 }
 
-// in-register bitonic sort of NC keys (compile-time network)
-template <int NC>
-__device__ __forceinline__ void bitonic_sort(uint32_t (&v)[NC]) {
+// two 16-bit lanes of one register, compared lane by lane (v_pk_min_u16 /
+// v_pk_max_u16)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(
+        __builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max_u16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(
+        __builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+// low 16 bits of lo, high 16 bits of hi
+__device__ __forceinline__ uint32_t halves(uint32_t lo, uint32_t hi) {
+    return (lo & 0xffffu) | (hi & 0xffff0000u);
+}
+
+// In-register bitonic sort (compile-time network) of NC 16-bit keys, two per
+// register: key i is in register i % R, low half for i < R, high half
+// otherwise (R = NC/2).  A comparator (i, i ^ stride) with stride < R pairs
+// the same halves of two registers, so one packed min and one packed max do
+// two comparators; both halves share the direction ((i & size) == 0 means
+// ascending) except in the merges of size R, where the low half ascends and
+// the high half descends.  stride == R (the first step of the last merge)
+// pairs the two halves of one register.
+// one step (merge size SIZE, comparator distance STRIDE) of the network,
+// then the rest by template recursion (nested loops this deep exceed the
+// compiler's full-unroll limit at NC = 128)
+template <int NC, int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_step_u16x2(uint32_t (&v)[NC / 2]) {
+    constexpr int R = NC / 2;
+    if constexpr (STRIDE == R) {
 #pragma unroll
-    for (int size = 2; size <= NC; size <<= 1) {
+        for (int r = 0; r < R; ++r) {
+            const uint32_t sw = __builtin_amdgcn_alignbit(v[r], v[r], 16);
+            v[r] = halves(pk_min_u16(v[r], sw), pk_max_u16(v[r], sw));
+        }
+    } else {
 #pragma unroll
-        for (int stride = size / 2; stride > 0; stride >>= 1) {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const uint32_t x = v[i], y = v[j];
-                    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
-                    if ((i & size) == 0) { v[i] = lo; v[j] = hi; }
-                    else { v[i] = hi; v[j] = lo; }
+        for (int r = 0; r < R; ++r) {
+            const int j = r ^ STRIDE;
+            if (j > r) {
+                const uint32_t mn = pk_min_u16(v[r], v[j]);
+                const uint32_t mx = pk_max_u16(v[r], v[j]);
+                if (SIZE == R) {
+                    v[r] = halves(mn, mx);
+                    v[j] = halves(mx, mn);
+                } else if ((r & SIZE) == 0) {
+                    v[r] = mn;
+                    v[j] = mx;
+                } else {
+                    v[r] = mx;
+                    v[j] = mn;
                 }
             }
         }
     }
+    if constexpr (STRIDE > 1) bitonic_step_u16x2<NC, SIZE, STRIDE / 2>(v);
+    else if constexpr (SIZE < NC) bitonic_step_u16x2<NC, SIZE * 2, SIZE>(v);
+}
+
+template <int NC>
+__device__ __forceinline__ void bitonic_sort_u16x2(uint32_t (&v)[NC / 2]) {
+    bitonic_step_u16x2<NC, 2, 1>(v);
 }
 
 // Sets of at most NC genes: one lane per set.  The lane gathers its hit
 // ranks, sorts them in registers and walks them in rank order.  Rows t >= k
 // of the gathered gene ids hold gene 0, so every load is in bounds; those
-// slots get the sentinel rank (sorted last) and are masked in the sums.
+// slots get the sentinel rank 0xffff (above every rank, as N <= 65535;
+// sorted last) and are masked in the sums.
 template <class T, int NC, class AF>
 __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
                                               const uint16_t *pos, AF &A,
                                               int64_t gp, int chunk, int lane) {
+    constexpr int R = NC / 2;
     const int slot = chunk * kWave + lane;
     const int sid = a.small_ids[slot];
     const int k = a.small_k[slot];
-    uint32_t q[NC];
-    const int32_t *gcol = a.small_gm + slot;
-    // opaque per call: keeps the compiler from hoisting 32 row offsets out of
-    // the work loop into (spilled) scalar registers
+    uint32_t v[R];
+    const uint16_t *gcol = a.small_gm + slot;
+    // opaque per call: keeps the compiler from hoisting the row offsets out
+    // of the work loop into (spilled) scalar registers
     int stride = a.small_stride;
     asm volatile("" : "+s"(stride));
-    // gather in groups of 8 so that only 8 loads (and addresses) are live
-#pragma unroll
-    for (int t0 = 0; t0 < NC; t0 += 8) {
-        int gg[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t r = pos[gg[u]];
-            q[t0 + u] = t0 + u < k ? r : 0xffffffffu;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    bitonic_sort<NC>(q);
+    // gather in groups of 8 so that only 8 loads (and addresses) are live;
+    // N_R is summed here, in set order (a double sum of at most 128 values
+    // of T, rounded to T below).  Each group takes its own opaque copy of k:
+    // the compiler would otherwise compute all NC masks t < k once and keep
+    // them live through the whole function (spills at NC = 128).
     double nr = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
+        int gg[8];
+        int kg = k;
+        asm volatile("" : "+v"(kg));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
+        uint32_t g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = pos[gg[u]];   // rows t >= k hold gene 0
         T x[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < k ? (int)q[t0 + u] : 0);
+        for (int u = 0; u < 8; ++u) x[u] = A((int)g[u]);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) nr += t0 + u < k ? (double)x[u] : 0.0;
-        __builtin_amdgcn_sched_barrier(0);
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u;
+            const uint32_t r = t < kg ? g[u] : 0xffffu;
+            if (t < R) v[t] = r;
+            else v[t - R] |= r << 16;
+            nr += t < kg ? (double)x[u] : 0.0;
+        }
+        // the group's results are final here: its loads, masks and values
+        // die before the next group starts
+        asm volatile("" : "+v"(nr));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) asm volatile("" : "+v"(v[(t0 + u) % R]));
     }
+    bitonic_sort_u16x2<NC>(v);
+    auto rank_at = [&](int t) -> int {
+        return t < R ? (int)(v[t] & 0xffffu) : (int)(v[t - R] >> 16);
+    };
     const HitShare<T> share((T)nr);
+    // opaque keys: the walk recomputes the addresses per group instead of
+    // keeping all NC of them live from the pass above (spills)
+#pragma unroll
+    for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
 
     const double decay = sid >= 0 ? (double)a.decay[sid] : 0.0;
     T best = T(0);
@@ -677,12 +740,14 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
         T x[8];
+        int kk = k;
+        asm volatile("" : "+v"(kk));
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < k ? (int)q[t0 + u] : 0);
+        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < kk ? rank_at(t0 + u) : 0);
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            es_candidates<T>(Sa, (double)share(x[u]), (int)q[t0 + u], t0 + u, decay,
-                             best, t0 + u < k);
+            es_candidates<T>(Sa, (double)share(x[u]), rank_at(t0 + u), t0 + u, decay,
+                             best, t0 + u < kk);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (k == 0) best = (T)((double)a.N * decay);
@@ -1055,8 +1120,12 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                         es_small_sets<T, 8>(a, pos, A, gp, chunk, lane);
                     else if (chunk < a.chunk_end16)
                         es_small_sets<T, 16>(a, pos, A, gp, chunk, lane);
-                    else
+                    else if (chunk < a.chunk_end32)
                         es_small_sets<T, 32>(a, pos, A, gp, chunk, lane);
+                    else if (chunk < a.chunk_end64)
+                        es_small_sets<T, 64>(a, pos, A, gp, chunk, lane);
+                    else
+                        es_small_sets<T, 128>(a, pos, A, gp, chunk, lane);
                 }
                 item = __builtin_amdgcn_readfirstlane(next);
             }

@@ -174,6 +174,10 @@ def test_rank_es_edge_sets(gpu):
             np.arange(N, dtype=np.int32), np.array([N - 1], np.int32),
             np.sort(rng.choice(N, 2999, replace=False)).astype(np.int32),
             np.sort(rng.choice(N, 5, replace=False)).astype(np.int32)]
+    # both sides of every size-class boundary of the set work split
+    sets += [np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
+             for k in (8, 9, 16, 17, 31, 32, 33, 63, 64, 65, 128, 129)]
+    sets += [np.arange(N - 64, N, dtype=np.int32), np.arange(64, dtype=np.int32)]
     off = np.zeros(len(sets) + 1, np.int32)
     off[1:] = np.cumsum([len(s) for s in sets])
     w["set_offsets"], w["set_genes"], w["G"] = off, np.concatenate(sets), len(sets)
