@@ -383,7 +383,8 @@ static bool fit(size_t region0, size_t npad, size_t region2, int lds_max, LdsPla
 
 template <class T>
 static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
-    pl->maxch = pick_maxch(N);
+    // N + 1: k_es keeps rank N free as a sentinel (|metric| 0, bit cleared)
+    pl->maxch = pick_maxch(N + 1);
     const size_t npad = (size_t)kRankThreads * pl->maxch;
     // bitmap words per wavefront: W = (maxch+1)/2 words per lane
     pl->nw = kWave * ((pl->maxch + 1) / 2);
@@ -392,9 +393,12 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     pl->rank.alds = sizeof(T) == 4 && fit(npad * 4, npad, counters, d->lds_max, &pl->rank);
     if (!pl->rank.alds && !fit(0, npad, counters, d->lds_max, &pl->rank))
         return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
+    // k_es: |metric| by rank and pos for genes/ranks 0..N (the sentinel
+    // included), not padded to the rank kernel's chunks
+    const size_t nes = (size_t)N + 1;
     for (int alds = (sizeof(T) == 4) ? 1 : 0; alds >= 0; --alds) {
         for (int ew = kRankWaves; ew >= 1; --ew) {
-            if (fit(alds ? npad * 4 : 0, npad, (size_t)ew * pl->nw * 4, d->lds_max, &pl->es)) {
+            if (fit(alds ? nes * 4 : 0, nes, (size_t)ew * pl->nw * 4, d->lds_max, &pl->es)) {
                 pl->es.alds = alds != 0;
                 pl->es.waves = ew;
                 return GSEA_OK;
