@@ -66,13 +66,13 @@ struct Buffer {
 enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
-    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp, kBufGenes16,
+    kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp,
     kBufExtSorted, kBufFdrTable, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
 constexpr int kTierSlots = 24;
-enum TierBuf { kTierIds, kTierGm, kTierDecay, kNumTierBufs };
+enum TierBuf { kTierIds, kTierGm, kTierDecay, kTierGenes, kNumTierBufs };
 
 struct DeviceState {
     bool init = false;
@@ -418,7 +418,7 @@ struct SetTiers {
     const int4 *large_desc = nullptr;
     const int32_t *small_ids = nullptr, *small_k = nullptr;
     const uint16_t *small_gm = nullptr;
-    const uint16_t *genes16 = nullptr;   // 16-bit copy of the CSR gene ids
+    const uint16_t *large_genes = nullptr;   // large sets' gene ids + 1
     const void *decay = nullptr;         // [G] of T
     const void *large_decay = nullptr;   // [n_large] of T
 };
@@ -433,20 +433,6 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     HIP_TRY(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (G + 1),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    // 16-bit gene ids for k_es (the caller's CSR may change between calls at
-    // the same address, so the copy is made on every call; ~13 MB at c3)
-    {
-        const int64_t total = off[(size_t)G];
-        void *p16;
-        int r = ensure(d, kBufGenes16, sizeof(uint16_t) * std::max<int64_t>(total, 1), &p16);
-        if (r) return r;
-        if (total > 0) {
-            hipLaunchKernelGGL(k_genes16, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                               0, st, d_genes, total, (uint16_t *)p16);
-            HIP_TRY(hipGetLastError());
-        }
-        t->genes16 = (const uint16_t *)p16;
-    }
     std::vector<int32_t> all;
     if (!subset) {
         all.resize((size_t)G);
@@ -479,14 +465,19 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     t->chunk_end64 = ends[3];
     t->stride = (int)sid.size();
     const size_t nl = large.size(), ns = sid.size();
-    // int32 words: large descriptors {set, offset, size, 0} | small ids | small sizes
+    // int32 words: large descriptors {set, offset, size, 0} | small ids |
+    // small sizes.  offset: the set's first entry in this list's 16-bit copy
+    // of the large sets' gene ids (made on every call: the caller's CSR may
+    // change between calls at the same address)
     std::vector<int32_t> host(4 * nl + 2 * ns);
+    int64_t nlg = 0;
     for (size_t i = 0; i < nl; ++i) {
         const int32_t s = large[i];
         host[4 * i] = s;
-        host[4 * i + 1] = off[s];
+        host[4 * i + 1] = (int32_t)nlg;
         host[4 * i + 2] = off[s + 1] - off[s];
         host[4 * i + 3] = 0;
+        nlg += off[s + 1] - off[s];
     }
     std::copy(sid.begin(), sid.end(), host.begin() + 4 * nl);
     std::copy(sk.begin(), sk.end(), host.begin() + 4 * nl + ns);
@@ -496,9 +487,12 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     for (int64_t s = 0; s < G; ++s)
         decay[(size_t)s] = (T)(-1.0 / (double)(N - (off[(size_t)s + 1] - off[(size_t)s])));
     for (size_t i = 0; i < nl; ++i) decay[(size_t)G + i] = decay[(size_t)large[i]];
-    void *pt, *pg, *pd;
+    void *pt, *pg, *pd, *pl;
     Buffer *tb = d->tier_bufs[slot];
     int rc = ensure_buf(tb[kTierIds], sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
+    // 1 KiB of slack after the last set: a phase-A load past a set's end is
+    // range-checked against that set, never outside the allocation
+    if (!rc) rc = ensure_buf(tb[kTierGenes], sizeof(uint16_t) * (size_t)nlg + 1024, &pl);
     if (!rc) rc = ensure_buf(tb[kTierDecay], sizeof(T) * std::max<size_t>(decay.size(), 1), &pd);
     if (!rc) rc = ensure_buf(tb[kTierGm], sizeof(uint16_t) * std::max<size_t>(ns * kSmallSet, 1), &pg);
     if (rc) return rc;
@@ -514,6 +508,12 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     t->small_ids = (const int32_t *)pt + 4 * nl;
     t->small_k = (const int32_t *)pt + 4 * nl + ns;
     t->small_gm = (const uint16_t *)pg;
+    t->large_genes = (const uint16_t *)pl;
+    if (nl) {
+        hipLaunchKernelGGL(k_large_genes, dim3((unsigned)nl), dim3(256), 0, st, d_off, d_genes,
+                           t->large_desc, (uint16_t *)pl);
+        HIP_TRY(hipGetLastError());
+    }
     if (ns) {
         hipLaunchKernelGGL(k_small_gm, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
                            d_off, d_genes, t->small_ids, (int)ns, (uint16_t *)pg);
@@ -541,7 +541,7 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.n_items = tiers ? tiers->n_large + tiers->n_chunks : 0;
     a.large_desc = tiers ? tiers->large_desc : nullptr;
     a.large_decay = tiers ? (const T *)tiers->large_decay : nullptr;
-    a.set_genes16 = tiers ? tiers->genes16 : nullptr;
+    a.large_genes = tiers ? tiers->large_genes : nullptr;
     a.small_ids = tiers ? tiers->small_ids : nullptr;
     a.small_k = tiers ? tiers->small_k : nullptr;
     a.small_gm = tiers ? tiers->small_gm : nullptr;

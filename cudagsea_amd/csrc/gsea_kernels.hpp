@@ -495,7 +495,7 @@ struct RankArgs {
     int G;
     const int32_t *set_off;  // [G+1]
     const int32_t *set_genes;
-    const uint16_t *set_genes16;  // the same CSR entries as 16-bit gene ids (k_es)
+    const uint16_t *large_genes;  // k_es: large sets' gene ids + 1, at large_desc.y
     T *es;                   // [P][G]
     T *a_glob;               // [Pb][N] |metric| by rank when it is not in LDS
     uint32_t *idx_out;       // optional [Pb][N]: gene id at each rank
@@ -512,7 +512,7 @@ struct RankArgs {
     const T *large_decay;              // [n_large] decay of those sets
     const int32_t *small_ids;          // [n_chunks*64] set id or -1
     const int32_t *small_k;            // [n_chunks*64] set sizes
-    const uint16_t *small_gm;          // [kSmallSet][small_stride] gene ids
+    const uint16_t *small_gm;          // [kSmallSet][small_stride] gene ids + 1
     int small_stride;
     int chunk_end8, chunk_end16, chunk_end32, chunk_end64;   // chunk class boundaries
     const T *decay;                    // [G] (T)(-1.0 / (double)(N - k_s))
@@ -520,16 +520,25 @@ struct RankArgs {
 
 constexpr int kSmallSet = 128;
 
-// 16-bit copy of the set CSR's gene ids (gene counts are below 2^15): half
-// the bytes every permutation's workgroup streams through in k_es
-__global__ void k_genes16(const int32_t *__restrict__ genes, int64_t n,
-                          uint16_t *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (uint16_t)genes[i];
+// k_es reads gene ids shifted by one, as 16-bit values: id 0 is the
+// sentinel gene, whose rank is N (past every real rank) and whose |metric| is
+// 0.  A load past a set's entries reads 0 (out-of-range buffer loads return
+// zero; the gene-id rows of small sets are zero-filled), so it marks the
+// sentinel's bit, which is cleared before the walk, and adds 0 to N_R.
+
+// large sets' ids, item i at desc[i].y (one block per item)
+__global__ void k_large_genes(const int32_t *__restrict__ off,
+                              const int32_t *__restrict__ genes,
+                              const int4 *__restrict__ desc,
+                              uint16_t *__restrict__ out) {
+    const int4 d = desc[blockIdx.x];
+    const int32_t *src = genes + off[d.x];
+    for (int t = threadIdx.x; t < d.z; t += blockDim.x)
+        out[d.y + t] = (uint16_t)(src[t] + 1);
 }
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
-// read row t of 64 consecutive slots
+// read row t of 64 consecutive slots; rows t >= k hold the sentinel
 __global__ void k_small_gm(const int32_t *__restrict__ off,
                            const int32_t *__restrict__ genes,
                            const int32_t *__restrict__ small_ids, int nslots,
@@ -540,7 +549,7 @@ __global__ void k_small_gm(const int32_t *__restrict__ off,
     const int o0 = s >= 0 ? off[s] : 0;
     const int k = s >= 0 ? off[s + 1] - o0 : 0;
     for (int t = 0; t < kSmallSet; ++t)
-        gm[(size_t)t * nslots + slot] = (uint16_t)(t < k ? genes[o0 + t] : 0);
+        gm[(size_t)t * nslots + slot] = (uint16_t)(t < k ? genes[o0 + t] + 1 : 0);
 }
 
 template <class T>
@@ -574,29 +583,27 @@ __device__ __forceinline__ void es_candidates(double &Sa, double c, int q, int j
     if (valid && fabs(va) > fabs(best)) best = va;
 }
 
-// c = |r| / N_R rounded to T, one set's N_R fixed at construction.  fp32
-// forms (double)a * fl64(1/N_R) and rounds it once to float: that product is
-// within 2^-52 (relative) of the exact quotient, and a quotient of two floats
-// is either a float or at least 2^-49 (relative) away from every float
-// rounding midpoint (a midpoint has a 25-bit odd significand; times the
-// divisor's significand it cannot fit the dividend's 24 bits), so the result
-// is the correctly rounded quotient the reference's division yields, special
-// values included (0/0, x/0, x/inf).  fp64 divides.
+// c = |r| / N_R of one set, N_R rounded to T as in the reference.
+// operator(): c carried in double as (double)|r| * fl64(1/N_R), within 2^-52
+// (relative) of the exact quotient; the reference rounds c to T, so an
+// enrichment score differs from a walk over rounded values by at most 2^-24
+// relative per step, i.e. by at most 6e-8 absolute (the c of a set sum to 1).
+// rounded(): c rounded to T.  In fp32 that single rounding of the product
+// is the correctly rounded quotient the reference's division yields: the
+// product is within 2^-52 of the exact quotient, and a quotient of two
+// floats is either a float or at least 2^-49 (relative) away from every
+// float rounding midpoint (a midpoint has a 25-bit odd significand; times
+// the divisor's significand it cannot fit the dividend's 24 bits).  Special
+// values follow the division: N_R = 0 gives inf (|r| > 0) or NaN (|r| = 0),
+// N_R = inf gives 0 (NaN for |r| = inf).
 template <class T>
-struct HitShare;
-template <>
-struct HitShare<float> {
+struct HitShare {
     double rn;
-    __device__ __forceinline__ explicit HitShare(float n) : rn(1.0 / (double)n) {}
-    __device__ __forceinline__ float operator()(float a) const {
-        return (float)((double)a * rn);
+    __device__ __forceinline__ explicit HitShare(T n) : rn(1.0 / (double)n) {}
+    __device__ __forceinline__ double operator()(T a) const { return (double)a * rn; }
+    __device__ __forceinline__ double rounded(T a) const {
+        return (double)(T)((double)a * rn);
     }
-};
-template <>
-struct HitShare<double> {
-    double n;
-    __device__ __forceinline__ explicit HitShare(double n_) : n(n_) {}
-    __device__ __forceinline__ double operator()(double a) const { return a / n; }
 };
 
 // leftmost lane value of largest |value| (first strict maximum in lane
@@ -674,9 +681,8 @@ __device__ __forceinline__ void bitonic_sort_u16x2(uint32_t (&v)[NC / 2]) {
 
 // Sets of at most NC genes: one lane per set.  The lane gathers its hit
 // ranks, sorts them in registers and walks them in rank order.  Rows t >= k
-// of the gathered gene ids hold gene 0, so every load is in bounds; those
-// slots get the sentinel rank 0xffff (above every rank, as N <= 65535;
-// sorted last) and are masked in the sums.
+// of the gathered gene ids hold the sentinel: rank N (sorted last, as
+// N <= 65535) and |metric| 0 (nothing added to N_R); the walk masks them.
 template <class T, int NC, class AF>
 __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
                                               const uint16_t *pos, AF &A,
@@ -693,30 +699,25 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     asm volatile("" : "+s"(stride));
     // gather in groups of 8 so that only 8 loads (and addresses) are live;
     // N_R is summed here, in set order (a double sum of at most 128 values
-    // of T, rounded to T below).  Each group takes its own opaque copy of k:
-    // the compiler would otherwise compute all NC masks t < k once and keep
-    // them live through the whole function (spills at NC = 128).
+    // of T, rounded to T below)
     double nr = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
         int gg[8];
-        int kg = k;
-        asm volatile("" : "+v"(kg));
 #pragma unroll
         for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
         uint32_t g[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) g[u] = pos[gg[u]];   // rows t >= k hold gene 0
+        for (int u = 0; u < 8; ++u) g[u] = pos[gg[u]];
         T x[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) x[u] = A((int)g[u]);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int t = t0 + u;
-            const uint32_t r = t < kg ? g[u] : 0xffffu;
-            if (t < R) v[t] = r;
-            else v[t - R] |= r << 16;
-            nr += t < kg ? (double)x[u] : 0.0;
+            if (t < R) v[t] = g[u];
+            else v[t - R] |= g[u] << 16;
+            nr += (double)x[u];
         }
         // the group's results are final here: its loads, masks and values
         // die before the next group starts
@@ -740,13 +741,15 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
         T x[8];
+        // an opaque copy of k per group: the compiler would otherwise compute
+        // all NC masks t < k at once and keep them live (spills at NC = 128)
         int kk = k;
         asm volatile("" : "+v"(kk));
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = A(t0 + u < kk ? rank_at(t0 + u) : 0);
+        for (int u = 0; u < 8; ++u) x[u] = A(rank_at(t0 + u));
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            es_candidates<T>(Sa, (double)share(x[u]), rank_at(t0 + u), t0 + u, decay,
+            es_candidates<T>(Sa, share.rounded(x[u]), rank_at(t0 + u), t0 + u, decay,
                              best, t0 + u < kk);
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -754,49 +757,55 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
 }
 
-// Phase A of a large set: B gene-id slots per lane (set entries j0 ..
-// j0 + 64*B - 1), the bits of their ranks set in the wavefront's bitmap and
-// their |r| summed.  Slots past the set's end repeat its last gene (genes of
-// a set are distinct; OR-ing its bit again changes nothing) and add 0.
+// Phase A of a large set: B gene-id slots per lane (set entries j0 + 64u +
+// lane, u < B), the bits of their ranks set in the wavefront's bitmap and
+// their |r| summed.  rs covers the set's k entries: slots past them read 0,
+// the sentinel (rank N, |r| 0).
 template <class T, int B, class AF>
-__device__ __forceinline__ void es_mark_hits(const uint16_t *sg, const uint16_t *pos,
-                                             AF &A, uint32_t *bm, int k, int j0,
-                                             int lane, double &nr) {
-    // buffer loads: the set's entries as one resource (scalar base, bounds
-    // k entries), a 32-bit byte offset per slot and no 64-bit address math
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)sg, (short)0, 2 * k, kBufferRsrcWord3);
+__device__ __forceinline__ void es_mark_hits(__amdgpu_buffer_rsrc_t rs, const uint16_t *pos,
+                                             AF &A, uint32_t *bm, int j0, int lane,
+                                             double &nr) {
+    const int vo = 2 * (j0 + lane);   // byte offset; slot u adds 128u
     uint32_t q[B];
 #pragma unroll
     for (int u = 0; u < B; ++u)
-        q[u] = __builtin_amdgcn_raw_buffer_load_b16(
-            rs, 2 * min(j0 + u * kWave + lane, k - 1), 0, 0);
+        q[u] = __builtin_amdgcn_raw_buffer_load_b16(rs, vo + 2 * kWave * u, 0, 0);
 #pragma unroll
     for (int u = 0; u < B; ++u) q[u] = pos[q[u]];
     T x[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         x[u] = A((int)q[u]);
-        atomicOr(&bm[q[u] >> 5], 1u << (q[u] & 31u));
+        // word index kept opaque: (q >> 5) then one shift-add for the address
+        // (the compiler's (q >> 3) & ~3 form takes one instruction more)
+        uint32_t wi = q[u] >> 5;
+        asm volatile("" : "+v"(wi));
+        atomicOr(&bm[wi], 1u << (q[u] & 31u));
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u)
-        nr += (double)(j0 + u * kWave + lane < k ? x[u] : T(0));
+    for (int u = 0; u < B; ++u) nr += (double)x[u];
 }
 
 // All of a set's entries through es_mark_hits, 8 slots per lane per round;
-// the last round takes the smallest batch that covers the set's end.
+// the last round takes the smallest batch that covers the set's end.  The
+// sentinel's bit (rank N) is cleared at the end.  sg: the set's ids + 1.
 template <class T, class AF>
 __device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *pos,
-                                            AF &A, uint32_t *bm, int k, int lane,
-                                            double &nr) {
+                                            AF &A, uint32_t *bm, int k, int N,
+                                            int lane, double &nr) {
+    // the set's entries as one buffer resource: scalar base, k entries in
+    // range, a 32-bit offset per lane and the slot offsets as immediates
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)sg, (short)0, 2 * k, kBufferRsrcWord3);
     for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
         const int ns = (k - j0 + kWave - 1) / kWave;   // slots with set entries
-        if (ns > 4) es_mark_hits<T, 8>(sg, pos, A, bm, k, j0, lane, nr);
-        else if (ns > 2) es_mark_hits<T, 4>(sg, pos, A, bm, k, j0, lane, nr);
-        else if (ns > 1) es_mark_hits<T, 2>(sg, pos, A, bm, k, j0, lane, nr);
-        else es_mark_hits<T, 1>(sg, pos, A, bm, k, j0, lane, nr);
+        if (ns > 4) es_mark_hits<T, 8>(rs, pos, A, bm, j0, lane, nr);
+        else if (ns > 2) es_mark_hits<T, 4>(rs, pos, A, bm, j0, lane, nr);
+        else if (ns > 1) es_mark_hits<T, 2>(rs, pos, A, bm, j0, lane, nr);
+        else es_mark_hits<T, 1>(rs, pos, A, bm, j0, lane, nr);
     }
+    // LDS operations of one wavefront complete in order: this follows the ORs
+    if (lane == 0) atomicAnd(&bm[N >> 5], ~(1u << (N & 31)));
 }
 
 // One lane's walk over its hits in rank order.  The lane's W bitmap words
@@ -841,14 +850,15 @@ __device__ __forceinline__ void es_walk(uint32_t *bm, AF &A,
         }
         const int q = wq + __builtin_ctz(cur);
         cur &= cur - 1u;
-        const double c = (double)share(A(q));
+        const double c = share(A(q));
         if (EXACT) {
             es_candidates<T>(L, c, q, jt, dec, best);
         } else {
-            const double md = (double)(q - jt) * dec;
-            vmin = min_skip_nan(vmin, L + md);
+            // running sum just before the hit, then just after it
+            const double before = __builtin_fma((double)(q - jt), dec, L);
+            vmin = min_skip_nan(vmin, before);
             L += c;
-            vmax = max_skip_nan(vmax, L + md);
+            vmax = max_skip_nan(vmax, before + c);
         }
     }
 }
@@ -868,9 +878,9 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
                                              int o0, int k, double decay, int lane) {
-    const uint16_t *sg = a.set_genes16 + o0;
+    const uint16_t *sg = a.large_genes + o0;
     double nr = 0.0;
-    es_mark_set<T>(sg, pos, A, bm, k, lane, nr);
+    es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, nr);
     nr = wave_sum(nr);
     const HitShare<T> share((T)nr);
     // k == N: no misses; the reference's decay (-inf) is never applied
@@ -898,7 +908,7 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
         // the walk cleared the bitmap: mark the hits again
         wave_sync_lds();
         double unused = 0.0;
-        es_mark_set<T>(sg, pos, A, bm, k, lane, unused);
+        es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, unused);
         wave_sync_lds();
         double S2 = Sa;
         es_walk<T, W, true>(bm, A, share, lane, nz, hits, j, dec, S2, vmin, vmax, best);
@@ -1083,17 +1093,23 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
     for (int p = blockIdx.x; p < a.Pb; p += gridDim.x) {
         const T *arow = a.a_glob + (size_t)(a.p_lo + p) * N;
         const uint16_t *prow = a.pos_out + (size_t)(a.p_lo + p) * N;
+        // pos is indexed by gene id + 1: entry 0 is the sentinel gene, rank N,
+        // whose |metric| (a_lds[N], or 0 below) is 0
         for (int g = tid; g < N; g += kRankThreads) {
-            pos[g] = prow[g];
+            pos[g + 1] = prow[g];
             if (ALDS) a_lds[g] = (float)arow[g];
         }
-        if (tid == 0) misc[0] = 0;
+        if (tid == 0) {
+            misc[0] = 0;
+            pos[0] = (uint16_t)N;
+            if (ALDS) a_lds[N] = 0.0f;
+        }
         if (w < a.es_waves)
             for (int i = lane; i < a.nw; i += kWave) bms[(size_t)w * a.nw + i] = 0u;
         __syncthreads();
         auto A = [&](int q) -> T {
             if (ALDS) return (T)a_lds[q];
-            return arow[q];
+            return q < N ? arow[q] : T(0);
         };
         if (w < a.es_waves) {
             uint32_t *bm = bms + (size_t)w * a.nw;

@@ -4,7 +4,8 @@ Bars: metrics and ranks bit-exact (integer/index work and the reference's
 exact float arithmetic); ES and NES within 1e-5 relative (|d| <= 1e-5*|x| +
 1e-6) in fp32 and fp64; NP and FWER are counts over permutations and must
 agree exactly except where an ES comparison is within rounding of a tie
-(counted and bounded); FDR within 1e-5 relative where NES agree.
+(each count difference needs an oracle comparison within the ES bar of a
+tie); FDR within 1e-5 relative where NES agree.
 """
 import numpy as np
 import pytest
@@ -187,15 +188,36 @@ def test_rank_es_edge_sets(gpu):
 
 
 # ----------------------------------------------------------- whole path ----
-def _compare_stats(got, want, P, es_got=None, es_want=None):
+def _tie_counts(es_want, P):
+    """Per set, from the oracle's ES [G][P]: the permutations whose |ES|
+    lies within the ES bar of the observed |ES| (NP compares those two), and
+    those whose extreme normalised score over all sets lies within it of the
+    set's NES (FWER compares those).  Where two values are only pinned to
+    within the bar, their comparison may go either way."""
+    e = es_want.astype(np.float64)
+    a = np.abs(e)
+    np_ties = (np.abs(a - a[:, :1]) <= ES_RTOL * a[:, :1] + ES_ATOL).sum(axis=1) - 1
+    z = e * (P / a.sum(axis=1))[:, None]
+    nes = z[:, 0]
+    ext = np.where(nes[:, None] > 0, z.max(axis=0)[None, :], z.min(axis=0)[None, :])
+    fw_ties = (np.abs(ext - nes[:, None]) <= ES_RTOL * np.abs(nes)[:, None] + ES_ATOL).sum(axis=1)
+    return np_ties, fw_ties
+
+
+def _compare_stats(got, want, P, es_want=None):
     ok_es = es_close(got[0], want[0])
     ok_nes = es_close(got[1], want[1])
     assert ok_es.all() and ok_nes.all(), (np.argwhere(~ok_es)[:3], np.argwhere(~ok_nes)[:3])
-    # counts: exact unless a comparison sits on a rounding tie
+    # counts: exact except for comparisons that sit within the ES bar of a tie
     d_np = np.abs(got[2].astype(np.float64) - want[2]) * P
     d_fw = np.abs(got[3].astype(np.float64) - want[3]) * P
-    assert (d_np <= 2.5).all() and (d_np > 0.5).mean() <= 0.01
-    assert (d_fw <= 2.5).all() and (d_fw > 0.5).mean() <= 0.01
+    if es_want is not None:
+        np_ties, fw_ties = _tie_counts(es_want, P)
+        assert (d_np <= np_ties + 0.5).all(), np.argwhere(d_np > np_ties + 0.5)[:5]
+        assert (d_fw <= fw_ties + 0.5).all(), np.argwhere(d_fw > fw_ties + 0.5)[:5]
+    else:
+        assert (d_np <= 2.5).all() and (d_np > 0.5).mean() <= 0.01
+        assert (d_fw <= 2.5).all() and (d_fw > 0.5).mean() <= 0.01
     fdr_ok = np.abs(got[4].astype(np.float64) - want[4]) <= 1e-3 + 1e-5 * np.abs(want[4])
     assert (fdr_ok | (np.isnan(got[4]) & np.isnan(want[4]))).mean() >= 0.99
 
@@ -210,27 +232,28 @@ def test_full_path_c1(gpu, dbl):
     got, es_got = gpu.run(w["X"], w["labels"], w["set_offsets"], w["set_genes"], P,
                           "stable_signal2noise", double=dbl, want_es=True)
     assert es_close(es_got, es_want).mean() > 0.9999
-    _compare_stats(got, want, P)
+    _compare_stats(got, want, P, es_want)
 
 
 @pytest.mark.parametrize("metric", oracle.METRICS)
 def test_full_path_every_metric_small(gpu, metric):
     w = small("c3", num_genes=1500, num_sets=120, num_perms=200)
-    want, _ = oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"], 200,
-                          metric, want_es=False)
+    want, es_want = oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"],
+                                200, metric)
     got = gpu.run(w["X"], w["labels"], w["set_offsets"], w["set_genes"], 200, metric)
-    _compare_stats(got, want, 200)
+    _compare_stats(got, want, 200, es_want)
 
 
 def test_full_path_flags(gpu):
     w = small("c1", num_perms=128)
     for asc in (False, True):
         for swap in (False, True):
-            want, _ = oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"],
-                                  128, "onepass_signal2noise", ascending=asc, swap=swap)
+            want, es_want = oracle.gsea(w["X"], w["labels"], w["set_offsets"],
+                                        w["set_genes"], 128, "onepass_signal2noise",
+                                        ascending=asc, swap=swap)
             got = gpu.run(w["X"], w["labels"], w["set_offsets"], w["set_genes"], 128,
                           "onepass_signal2noise", ascending=asc, swap_labels=swap)
-            _compare_stats(got, want, 128)
+            _compare_stats(got, want, 128, es_want)
 
 
 def test_gsea_r_mirror_and_dump(gpu, tmp_path):
@@ -249,7 +272,7 @@ def test_gsea_r_mirror_and_dump(gpu, tmp_path):
     want, es_want = oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"],
                                 64, "stable_signal2noise")
     got = res.to_numpy().T.astype(np.float32)
-    _compare_stats(got, want, 64)
+    _compare_stats(got, want, 64, es_want)
     dump = gpu.read_dump(prefix + "_%d_64_32.es" % w["G"])
     assert es_close(dump, es_want).mean() > 0.9999
 
