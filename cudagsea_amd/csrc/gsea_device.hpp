@@ -189,9 +189,40 @@ __device__ __forceinline__ V dpp_move(V v) {
     }
 }
 
+// dpp_move_bc<CTRL>(v): every lane takes v from the lane CTRL names, lanes
+// with no such source get 0 (bound_ctrl), all rows are written: one
+// v_mov_b32_dpp per 32 bits, no initialisation of the destination.
+// dpp_move_any<CTRL, ROWS>(v): the same for the rows selected by ROWS; the
+// lanes of the other rows hold unspecified values (for reductions that
+// never read them).
+template <int CTRL, int ROWS, bool BC, class V>
+__device__ __forceinline__ V dpp_mov_raw(V v) {
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8, "32- or 64-bit lane values");
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(
+            V, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, ROWS, 0xf, BC));
+    } else {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, ROWS,
+                                                               0xf, BC);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL,
+                                                               ROWS, 0xf, BC);
+        return __builtin_bit_cast(V, lo | (hi << 32));
+    }
+}
+template <int CTRL, class V>
+__device__ __forceinline__ V dpp_move_bc(V v) {
+    return dpp_mov_raw<CTRL, 0xf, true>(v);
+}
+template <int CTRL, int ROWS, class V>
+__device__ __forceinline__ V dpp_move_any(V v) {
+    return dpp_mov_raw<CTRL, ROWS, false>(v);
+}
+
 constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
               kDppRowShr8 = 0x118, kDppWaveShr1 = 0x138, kDppRowBcast15 = 0x142,
-              kDppRowBcast31 = 0x143;
+              kDppRowBcast31 = 0x143, kDppQuadXor1 = 0xb1, kDppQuadXor2 = 0x4e,
+              kDppRowHalfMirror = 0x141, kDppRowMirror = 0x140;
 
 // value of lane l, as a wave-uniform value
 template <class V>
@@ -214,10 +245,10 @@ __device__ __forceinline__ V read_lane(V v, int l) {
 // op(0, x) must be x for the values that occur.
 template <class V, class Op>
 __device__ __forceinline__ V wave_incl_scan_op(V x, Op op) {
-    x = op(dpp_move<kDppRowShr1, 0xf>(x), x);
-    x = op(dpp_move<kDppRowShr2, 0xf>(x), x);
-    x = op(dpp_move<kDppRowShr4, 0xf>(x), x);
-    x = op(dpp_move<kDppRowShr8, 0xf>(x), x);
+    x = op(dpp_move_bc<kDppRowShr1>(x), x);
+    x = op(dpp_move_bc<kDppRowShr2>(x), x);
+    x = op(dpp_move_bc<kDppRowShr4>(x), x);
+    x = op(dpp_move_bc<kDppRowShr8>(x), x);
     x = op(dpp_move<kDppRowBcast15, 0xa>(x), x);
     x = op(dpp_move<kDppRowBcast31, 0xc>(x), x);
     return x;
@@ -228,29 +259,37 @@ __device__ __forceinline__ V wave_incl_scan(V x) {
     return wave_incl_scan_op(x, [](V l, V r) { return l + r; });
 }
 
-// sum over the 64 lanes in a fixed order, the same value on every lane
+// sum over the 64 lanes in a fixed order, the same value on every lane: a
+// butterfly within each row (lanes xor 1, xor 2, the mirrored half-row, the
+// mirrored row), then the row sums carried into lane 63 by the broadcasts
 template <class V>
 __device__ __forceinline__ V wave_sum(V v) {
-    return read_lane(wave_incl_scan(v), kWave - 1);
+    v = dpp_move_any<kDppQuadXor1, 0xf>(v) + v;
+    v = dpp_move_any<kDppQuadXor2, 0xf>(v) + v;
+    v = dpp_move_any<kDppRowHalfMirror, 0xf>(v) + v;
+    v = dpp_move_any<kDppRowMirror, 0xf>(v) + v;
+    v = dpp_move_any<kDppRowBcast15, 0xa>(v) + v;
+    v = dpp_move_any<kDppRowBcast31, 0xc>(v) + v;
+    return read_lane(v, kWave - 1);
 }
 
-// op over all 64 lanes for an idempotent, commutative op (min, max),
-// returned on every lane
+// op over all 64 lanes for an associative, commutative op (min, max),
+// returned on every lane: the butterfly of wave_sum
 template <class V, class Op>
 __device__ __forceinline__ V wave_reduce_idem(V x, Op op) {
-    x = op(dpp_move<kDppRowShr1, 0xf, V, true>(x), x);
-    x = op(dpp_move<kDppRowShr2, 0xf, V, true>(x), x);
-    x = op(dpp_move<kDppRowShr4, 0xf, V, true>(x), x);
-    x = op(dpp_move<kDppRowShr8, 0xf, V, true>(x), x);
-    x = op(dpp_move<kDppRowBcast15, 0xa, V, true>(x), x);
-    x = op(dpp_move<kDppRowBcast31, 0xc, V, true>(x), x);
+    x = op(dpp_move_any<kDppQuadXor1, 0xf>(x), x);
+    x = op(dpp_move_any<kDppQuadXor2, 0xf>(x), x);
+    x = op(dpp_move_any<kDppRowHalfMirror, 0xf>(x), x);
+    x = op(dpp_move_any<kDppRowMirror, 0xf>(x), x);
+    x = op(dpp_move_any<kDppRowBcast15, 0xa>(x), x);
+    x = op(dpp_move_any<kDppRowBcast31, 0xc>(x), x);
     return read_lane(x, kWave - 1);
 }
 
 // exclusive prefix sum over lanes in lane order (lane 0 gets 0)
 template <class V>
 __device__ __forceinline__ V wave_excl_scan(V v) {
-    return dpp_move<kDppWaveShr1, 0xf>(wave_incl_scan(v));
+    return dpp_move_bc<kDppWaveShr1>(wave_incl_scan(v));
 }
 
 }  // namespace gsea

@@ -584,22 +584,35 @@ __device__ __forceinline__ void es_candidates(double &Sa, double c, int q, int j
 }
 
 // c = |r| / N_R of one set, N_R rounded to T as in the reference.
-// operator(): c carried in double as (double)|r| * fl64(1/N_R), within 2^-52
-// (relative) of the exact quotient; the reference rounds c to T, so an
+// operator(): c carried in double as (double)|r| * rn with rn within one ulp
+// of 1/N_R, i.e. within 1.5 * 2^-52 (relative) of the exact quotient; the reference rounds c to T, so an
 // enrichment score differs from a walk over rounded values by at most 2^-24
 // relative per step, i.e. by at most 6e-8 absolute (the c of a set sum to 1).
 // rounded(): c rounded to T.  In fp32 that single rounding of the product
 // is the correctly rounded quotient the reference's division yields: the
-// product is within 2^-52 of the exact quotient, and a quotient of two
+// product is within 1.5 * 2^-52 of the exact quotient, and a quotient of two
 // floats is either a float or at least 2^-49 (relative) away from every
 // float rounding midpoint (a midpoint has a 25-bit odd significand; times
 // the divisor's significand it cannot fit the dividend's 24 bits).  Special
 // values follow the division: N_R = 0 gives inf (|r| > 0) or NaN (|r| = 0),
 // N_R = inf gives 0 (NaN for |r| = inf).
+// fl64(1/n) to within one ulp for finite n > 0: v_rcp_f64 and two Newton
+// steps (the error squares per step); 0, inf and NaN take the division
+__device__ __forceinline__ double recip(double n) {
+    if (n > 0.0 && n < __builtin_inf()) {
+        double r = __builtin_amdgcn_rcp(n);
+        double e = __builtin_fma(-n, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-n, r, 1.0);
+        return __builtin_fma(r, e, r);
+    }
+    return 1.0 / n;
+}
+
 template <class T>
 struct HitShare {
     double rn;
-    __device__ __forceinline__ explicit HitShare(T n) : rn(1.0 / (double)n) {}
+    __device__ __forceinline__ explicit HitShare(T n) : rn(recip((double)n)) {}
     __device__ __forceinline__ double operator()(T a) const { return (double)a * rn; }
     __device__ __forceinline__ double rounded(T a) const {
         return (double)(T)((double)a * rn);
@@ -830,6 +843,17 @@ __device__ __forceinline__ double max_skip_nan(double a, double b) {
     asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
+// the same for float (operands are quiet NaN or arithmetic results)
+__device__ __forceinline__ float min_skip_nan(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float max_skip_nan(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 
 template <class T, int W, bool EXACT, class AF>
 __device__ __forceinline__ void es_walk(uint32_t *bm, AF &A,
@@ -900,8 +924,8 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
     T best = T(0);
     es_walk<T, W, false>(bm, A, share, lane, nz, hits, j, dec, L, vmin, vmax, best);
     const double Sa = wave_excl_scan(L);
-    const T hi = wave_reduce_idem((T)(Sa + vmax), [](T x, T y) { return fmax(x, y); });
-    const T lo = wave_reduce_idem((T)(Sa + vmin), [](T x, T y) { return fmin(x, y); });
+    const T hi = wave_reduce_idem((T)(Sa + vmax), [](T x, T y) { return max_skip_nan(x, y); });
+    const T lo = wave_reduce_idem((T)(Sa + vmin), [](T x, T y) { return min_skip_nan(x, y); });
     const T ah = hi == hi ? fabs(hi) : T(0);
     const T al = lo == lo ? fabs(lo) : T(0);
     if (ah == al && ah != T(0)) {
@@ -1124,12 +1148,14 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                 int next = 0;
                 if (lane == 0) next = atomicAdd(&misc[0], 1);
                 if (item < a.n_large) {
-                    const int4 d = a.large_desc[item];
-                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp,
-                                             __builtin_amdgcn_readfirstlane(d.x),
-                                             __builtin_amdgcn_readfirstlane(d.y),
-                                             __builtin_amdgcn_readfirstlane(d.z),
-                                             (double)a.large_decay[item], lane);
+                    // read-only tables through the constant address space:
+                    // scalar loads of the uniform descriptor and decay
+                    const __attribute__((address_space(4))) int *d =
+                        (const __attribute__((address_space(4))) int *)(a.large_desc + item);
+                    const T dec = *((const __attribute__((address_space(4))) T *)
+                                        a.large_decay + item);
+                    es_large_set<T, ALDS, W>(a, pos, A, bm, gp, d[0], d[1], d[2], (double)dec,
+                                             lane);
                 } else {
                     const int chunk = item - a.n_large;
                     if (chunk < a.chunk_end8)
