@@ -1,5 +1,6 @@
 """ctypes front of the CPU oracle (oracle/_build/liboracle.so) and of the
-reference build (oracle/_ref/libgsea_ref.so).
+reference's permutation generator built from its own header
+(oracle/_ref/libgsea_ref.so, rngpu/rngpu.hpp compiled by g++ as shipped).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
 checker, and by bench.py's cpu_baseline leg.  The product (cudagsea_amd/) never
@@ -12,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# GSEA_ORACLE_LIB selects another build of the oracle (used to check that the
+# tests fail on an oracle with a planted mistake)
+LIB_PATH = os.environ.get("GSEA_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libgsea_ref.so")
 REF_SOURCES = "/root/reference/cudaGSEA/src/include"
 
@@ -50,6 +53,9 @@ def lib():
         L.orc_hash32.restype = ctypes.c_uint32
         L.orc_hash32.argtypes = [ctypes.c_uint32]
         L.orc_permute_labels.argtypes = [_P, _I64, ctypes.c_uint64, _P]
+        L.orc_kiss_init.argtypes = [ctypes.c_uint32, _P]
+        L.orc_kiss_next.restype = ctypes.c_uint32
+        L.orc_kiss_next.argtypes = [_P]
         L.orc_log_f64.restype = ctypes.c_double
         L.orc_log_f64.argtypes = [ctypes.c_double]
         L.orc_log_f32.restype = ctypes.c_float
@@ -88,12 +94,6 @@ def ref():
         R.ref_hash32.restype = ctypes.c_uint32
         R.ref_hash32.argtypes = [ctypes.c_uint32]
         R.ref_kiss_stream.argtypes = [ctypes.c_uint32, _P, _I64]
-        for sfx in ("f32", "f64"):
-            f = getattr(R, "ref_correlate_" + sfx)
-            f.argtypes = [_P, _P, _I64, _I64, _I64, _I64, ctypes.c_int, _P]
-            f.restype = ctypes.c_int
-            getattr(R, "ref_final_statistics_" + sfx).argtypes = [
-                _P, _I64, _I64, _P]
         _ref = R
     return _ref
 
@@ -106,6 +106,19 @@ def _dt(double):
     return np.float64 if double else np.float32
 
 
+def kiss_stream(seed, n, use_ref=False):
+    """The first n draws of the generator seeded with `seed`."""
+    out = np.empty(n, dtype=np.uint32)
+    if use_ref:
+        ref().ref_kiss_stream(seed, _ptr(out), n)
+    else:
+        st = (ctypes.c_uint32 * 5)()
+        lib().orc_kiss_init(seed, st)
+        for i in range(n):
+            out[i] = lib().orc_kiss_next(st)
+    return out
+
+
 def permute_labels(labels, perm, use_ref=False):
     labels = np.ascontiguousarray(labels, dtype=np.uint8)
     out = np.empty_like(labels)
@@ -115,8 +128,7 @@ def permute_labels(labels, perm, use_ref=False):
     return out
 
 
-def correlate(X, labels, P, metric, shift=0, sample_major=False, double=False,
-              use_ref=False):
+def correlate(X, labels, P, metric, shift=0, sample_major=False, double=False):
     """Metric matrix [P][N].  X is [N][S] (gene-major) unless sample_major."""
     dt = _dt(double)
     X = np.ascontiguousarray(X, dtype=dt)
@@ -128,16 +140,8 @@ def correlate(X, labels, P, metric, shift=0, sample_major=False, double=False,
     mid = METRICS.index(metric) if isinstance(metric, str) else int(metric)
     out = np.empty((P, N), dtype=dt)
     sfx = "f64" if double else "f32"
-    if use_ref:
-        if sample_major:
-            X = np.ascontiguousarray(X.T)
-        rc = getattr(ref(), "ref_correlate_" + sfx)(
-            _ptr(X), _ptr(labels), N, S, P, shift, mid, _ptr(out))
-        assert rc == 0
-    else:
-        getattr(lib(), "orc_correlate_" + sfx)(
-            _ptr(X), _ptr(labels), N, S, P, shift, mid, int(sample_major),
-            _ptr(out))
+    getattr(lib(), "orc_correlate_" + sfx)(
+        _ptr(X), _ptr(labels), N, S, P, shift, mid, int(sample_major), _ptr(out))
     return out
 
 
@@ -167,17 +171,14 @@ def scores(sorted_keys, index, offsets, genes):
     return es
 
 
-def final_statistics(es, use_ref=False):
+def final_statistics(es):
     """Returns (stats [5][G], normalised es)."""
     es = np.array(es, copy=True, order="C")
     double = es.dtype == np.float64
     G, P = es.shape
     out = np.empty((5, G), dtype=es.dtype)
     sfx = "f64" if double else "f32"
-    if use_ref:
-        getattr(ref(), "ref_final_statistics_" + sfx)(_ptr(es), G, P, _ptr(out))
-    else:
-        getattr(lib(), "orc_final_statistics_" + sfx)(_ptr(es), G, P, _ptr(out))
+    getattr(lib(), "orc_final_statistics_" + sfx)(_ptr(es), G, P, _ptr(out))
     return out, es
 
 

@@ -1,19 +1,24 @@
-"""Write tests/golden/ref_vectors.npz from the reference's own arithmetic.
+"""Write tests/golden/ref_vectors.npz from the reference's own permutation
+generator.
 
 Runs only where the reference sources exist: oracle/ref/Makefile compiles
-ref_driver.cpp against the reference headers under /root/reference into
-oracle/_ref/libgsea_ref.so; this script calls that library and stores inputs
-and outputs (data only) so the tests can pin the oracle without the
-reference present.
+oracle/ref/ref_rng.cpp with plain g++ against the reference header
+cudaGSEA/src/include/rngpu/rngpu.hpp (as shipped, nothing forced in or
+stubbed) into oracle/_ref/libgsea_ref.so; this script calls that library and
+stores inputs and outputs (data only) so the tests can pin the oracle without
+the reference present.
 
 Vectors:
-  perm_S<S>      label permutations of arange(S) for perm ids PERMS (RNG +
-                 Fisher-Yates, rngpu.hpp)
-  metric_X_f32/f64, metric_labels, metric_<m>_f32/f64
-                 the 14 metrics (functors.cuh) over permutations 0..7 of a
-                 small gene-major table, reference loop order
-  stats_in_f32/f64, stats_out_f32/f64, stats_norm_f32/f64
-                 final_statistics (statistics.cuh) on a random ES matrix
+  perm_ids, perm_S<S>   label permutations of arange(S) (mod 251 above 255)
+                        for the global permutation ids perm_ids (seed
+                        perm + 42, fast KISS + Fisher-Yates, rngpu.hpp)
+  kiss_seeds, kiss      the first 64 draws of the generator for each seed
+  hash_in, hash_out     the seeding hash (rngpu.hpp:27-38)
+
+The reference's metric functors, enrichment scores and statistics are not
+built here (they need nvcc's keywords, the CUDA runtime or CUB); the tests
+pin those oracle functions by closed forms, float64 brute force and
+invariants instead.
 """
 import os
 import sys
@@ -27,6 +32,7 @@ import oracle  # noqa: E402
 
 PERMS = [1, 2, 3, 17, 255, 4096, 65535, 1000003]
 SIZES = [2, 3, 10, 40, 200, 500]
+SEEDS = [0, 1, 43, 44, 4138, 4294967295]
 
 
 def main():
@@ -37,32 +43,12 @@ def main():
             np.arange(S) % 251).astype(np.uint8)
         out["perm_S%d" % S] = np.stack(
             [oracle.permute_labels(base, p, use_ref=True) for p in PERMS])
-
-    rng = np.random.default_rng(7)
-    N, S = 48, 14
-    X = np.exp(5 + rng.standard_normal((N, S)))
-    X[3, :] = 11.0                        # constant gene
-    X[5, :] = 0.0                         # all-zero gene (0/0 metrics)
-    X[7, ::2] = 1e4 + 1e-2 * rng.standard_normal(S // 2)  # cancellation
-    X[7, 1::2] = 1e4
-    labels = np.array([0, 1] * (S // 2), dtype=np.uint8)
-    out["metric_labels"] = labels
-    for dbl, sfx in ((False, "f32"), (True, "f64")):
-        Xt = X.astype(np.float64 if dbl else np.float32)
-        out["metric_X_" + sfx] = Xt
-        for m in oracle.METRICS:
-            out["metric_%s_%s" % (m, sfx)] = oracle.correlate(
-                Xt, labels, 8, m, double=dbl, use_ref=True)
-
-    es = (rng.standard_normal((12, 64)) * 0.3)
-    es[2, :] = 0.0                        # all-zero set row: NaN factor
-    es[4, 0] = 0.9
-    for dbl, sfx in ((False, "f32"), (True, "f64")):
-        e = es.astype(np.float64 if dbl else np.float32)
-        st, norm = oracle.final_statistics(e, use_ref=True)
-        out["stats_in_" + sfx] = e
-        out["stats_out_" + sfx] = st
-        out["stats_norm_" + sfx] = norm
+    out["kiss_seeds"] = np.array(SEEDS, dtype=np.uint32)
+    out["kiss"] = np.stack([oracle.kiss_stream(s, 64, use_ref=True) for s in SEEDS])
+    hin = np.array([0, 1, 42, 43, 12345, 0x7fffffff, 0xffffffff], dtype=np.uint32)
+    out["hash_in"] = hin
+    out["hash_out"] = np.array([oracle.ref().ref_hash32(int(x)) for x in hin],
+                               dtype=np.uint32)
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
