@@ -36,6 +36,13 @@ class gsea_extension(ctypes.Structure):
                 ("nes_tol", ctypes.c_double)]
 
 
+class gsea_tuning(ctypes.Structure):
+    _fields_ = [("batch_perms", ctypes.c_int64), ("reserved", ctypes.c_int64 * 7)]
+
+    def as_dict(self):
+        return {"batch_perms": self.batch_perms}
+
+
 class gsea_timings(ctypes.Structure):
     _fields_ = [("ms_lists", ctypes.c_double), ("ms_metric", ctypes.c_double),
                 ("ms_rank_es", ctypes.c_double), ("ms_stats", ctypes.c_double),
@@ -59,6 +66,8 @@ SIGNATURES = {
     "gsea_metric_name": (ctypes.c_char_p, [_I]),
     "gsea_last_error": (ctypes.c_char_p, []),
     "gsea_version": (ctypes.c_char_p, []),
+    "gsea_set_tuning": (_I, [_P]),
+    "gsea_get_tuning": (_I, [_P]),
     "gsea_device_count": (_I, [ctypes.POINTER(_I)]),
     "gsea_get_device": (_I, [ctypes.POINTER(_I)]),
     "gsea_set_device": (_I, [_I]),

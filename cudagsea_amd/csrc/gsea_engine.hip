@@ -22,6 +22,7 @@
 
 #include "gsea_hip.h"
 #include "gsea_kernels.hpp"
+#include "gsea_launch.hpp"
 
 namespace gsea {
 
@@ -86,6 +87,7 @@ struct DeviceState {
 
 static std::mutex g_mu;
 static DeviceState g_dev[64];
+static gsea_tuning g_tune = {};
 
 static int current_device(int *dev) {
     HIP_TRY(hipGetDevice(dev));
@@ -223,33 +225,6 @@ static int prepare_labels(const uint8_t *labels, int64_t S, int flags,
     return GSEA_OK;
 }
 
-template <class T, int KIND, int GT>
-static hipError_t launch_metric_gt(const MetricArgs<T> &a, bool xlds,
-                                   hipStream_t st) {
-    const int gx = (a.N + GT - 1) / GT;
-    const int gy = (a.Pb + a.ppb - 1) / a.ppb;
-    // X tile in LDS: one 16-byte read per sample, 4 fp32 or 2 fp64 genes
-    constexpr int GPL = sizeof(T) == 4 ? 4 : 2;
-    if (xlds) {
-        const size_t lds = (size_t)a.S * GT * sizeof(T);
-        hipLaunchKernelGGL((k_metric<T, KIND, GT, true, GPL>), dim3(gx, gy), dim3(256),
-                           lds, st, a);
-    } else {
-        hipLaunchKernelGGL((k_metric<T, KIND, GT, false, 1>), dim3(gx, gy), dim3(256),
-                           0, st, a);
-    }
-    return hipGetLastError();
-}
-
-template <class T, int KIND>
-static hipError_t launch_metric_kind(const MetricArgs<T> &a, int gt, bool xlds,
-                                     hipStream_t st) {
-    if (!xlds) return launch_metric_gt<T, KIND, 64>(a, false, st);
-    if (gt == 64) return launch_metric_gt<T, KIND, 64>(a, true, st);
-    if (gt == 32) return launch_metric_gt<T, KIND, 32>(a, true, st);
-    return launch_metric_gt<T, KIND, 16>(a, true, st);
-}
-
 // X tile width: the widest of 64/32/16 genes whose [S][GT] tile fits 64 KiB
 // (two workgroups per CU); otherwise X is read from global memory.
 template <class T>
@@ -290,67 +265,8 @@ static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
     int64_t ppb = (Pb * gx + target - 1) / target;
     ppb = std::max<int64_t>(32, std::min<int64_t>(512, (ppb + 31) / 32 * 32));
     a.ppb = (int)ppb;
-    hipError_t e;
-    switch (acc_kind(metric)) {
-    case kNaive: e = launch_metric_kind<T, kNaive>(a, gt, xlds, st); break;
-    case kKahan: e = launch_metric_kind<T, kKahan>(a, gt, xlds, st); break;
-    case kOnepass: e = launch_metric_kind<T, kOnepass>(a, gt, xlds, st); break;
-    case kTwopass: e = launch_metric_kind<T, kTwopass>(a, gt, xlds, st); break;
-    case kKnuth: e = launch_metric_kind<T, kKnuth>(a, gt, xlds, st); break;
-    default: e = launch_metric_kind<T, kOverkill>(a, gt, xlds, st); break;
-    }
-    HIP_TRY(e);
+    HIP_TRY(launch_metric_kernel<T>(a, acc_kind(metric), gt, xlds, st));
     return GSEA_OK;
-}
-
-// launches with more than the default 64 KiB of dynamic LDS opt in first
-template <class T, bool ALDS, int MAXCH>
-static hipError_t launch_rank_mc(const RankArgs<T> &a, size_t lds, int grid,
-                                 hipStream_t st) {
-    auto kern = k_rank<T, ALDS, MAXCH>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kRankThreads), lds, st, a);
-    return hipGetLastError();
-}
-
-template <class T, bool ALDS, int W>
-static hipError_t launch_es_w(const RankArgs<T> &a, size_t lds, int grid,
-                              hipStream_t st) {
-    auto kern = k_es<T, ALDS, W>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kRankThreads), lds, st, a);
-    return hipGetLastError();
-}
-
-// MAXCH chunks of 64 keys per wavefront; k_es lanes own (MAXCH+1)/2 words
-template <class T, bool RANK, bool ALDS>
-static hipError_t launch_by_maxch(const RankArgs<T> &a, int maxch, size_t lds,
-                                  int grid, hipStream_t st) {
-#define GSEA_CASE(C)                                                         \
-    case C:                                                                  \
-        return RANK ? launch_rank_mc<T, ALDS, C>(a, lds, grid, st)           \
-                    : launch_es_w<T, ALDS, (C + 1) / 2>(a, lds, grid, st);
-    switch (maxch) {
-        GSEA_CASE(1) GSEA_CASE(2) GSEA_CASE(4) GSEA_CASE(8) GSEA_CASE(12)
-        GSEA_CASE(16) GSEA_CASE(20) GSEA_CASE(24) GSEA_CASE(28)
-    default:
-        return RANK ? launch_rank_mc<T, ALDS, 32>(a, lds, grid, st)
-                    : launch_es_w<T, ALDS, 16>(a, lds, grid, st);
-    }
-#undef GSEA_CASE
-}
-
-static int pick_maxch(int64_t N) {
-    const int need = (int)((N + kRankThreads - 1) / kRankThreads);
-    for (int c : {1, 2, 4, 8, 12, 16, 20, 24, 28, 32})
-        if (c >= need) return c;
-    return 32;
 }
 
 // LDS layouts.  k_rank: [keys (fp32 only)] | ids | 16 x 256 counters | misc.
@@ -364,8 +280,9 @@ struct LdsPlan {
 };
 
 struct RankPlan {
-    int maxch = 1;
-    int nw = 0;
+    int maxch = 1;   // k_rank: chunks of 64 keys per wavefront
+    int W = 1;       // k_es: bitmap words per lane
+    int nw = 0;      // k_es: bitmap words per wavefront (64 W)
     LdsPlan rank, es;
 };
 
@@ -383,11 +300,14 @@ static bool fit(size_t region0, size_t npad, size_t region2, int lds_max, LdsPla
 
 template <class T>
 static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
-    // N + 1: k_es keeps rank N free as a sentinel (|metric| 0, bit cleared)
-    pl->maxch = pick_maxch(N + 1);
+    pl->maxch = pick_maxch(N);
+    if (pl->maxch == 0)
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
     const size_t npad = (size_t)kRankThreads * pl->maxch;
-    // bitmap words per wavefront: W = (maxch+1)/2 words per lane
-    pl->nw = kWave * ((pl->maxch + 1) / 2);
+    // k_es's bitmap holds ranks 0..N: rank N is the sentinel's (|metric| 0,
+    // bit cleared before the walk), so N + 1 bits, W words per lane
+    pl->W = (int)((N + 1 + 32 * kWave - 1) / (32 * kWave));
+    pl->nw = kWave * pl->W;
     const size_t counters = (size_t)kRankWaves * 256 * 4;
     // k_rank keeps fp32 keys in LDS; fp64 keys are recomputed from r
     pl->rank.alds = sizeof(T) == 4 && fit(npad * 4, npad, counters, d->lds_max, &pl->rank);
@@ -533,6 +453,7 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.G = (int)G; a.set_off = d_off; a.set_genes = d_genes; a.es = es_pg;
     a.asc = (flags & GSEA_FLAG_ASCENDING) ? 1 : 0;
     a.nw = pl.nw;
+    a.W = pl.W;
     a.a_glob = nullptr;
     a.pos_out = nullptr;
     a.p_lo = 0;
@@ -579,9 +500,7 @@ static int launch_es_rows(int64_t N, int64_t rows_cap, int64_t perm_base,
     a.es_waves = pl.es.waves;
     a.off_ids = pl.es.off_ids; a.off_cnt = pl.es.off_cnt; a.off_misc = pl.es.off_misc;
     const int grid = (int)(row_hi - row_lo);
-    hipError_t e = pl.es.alds
-        ? launch_by_maxch<T, false, true>(a, pl.maxch, pl.es.lds, grid, st)
-        : launch_by_maxch<T, false, false>(a, pl.maxch, pl.es.lds, grid, st);
+    hipError_t e = launch_es_kernel<T>(a, pl.es.alds, pl.es.lds, grid, st);
     HIP_TRY(e);
     return GSEA_OK;
 }
@@ -612,9 +531,7 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
     const int grid = (int)Pb;
     a.es_waves = kRankWaves;
     a.off_ids = pl.rank.off_ids; a.off_cnt = pl.rank.off_cnt; a.off_misc = pl.rank.off_misc;
-    hipError_t e = pl.rank.alds
-        ? launch_by_maxch<T, true, true>(a, pl.maxch, pl.rank.lds, grid, st)
-        : launch_by_maxch<T, true, false>(a, pl.maxch, pl.rank.lds, grid, st);
+    hipError_t e = launch_rank_kernel<T>(a, pl.rank.alds, pl.maxch, pl.rank.lds, grid, st);
     HIP_TRY(e);
     if (G > 0)
         return launch_es_rows<T>(N, Pb, perm_base, d_off, d_genes, G, flags, es_pg, d, st,
@@ -763,8 +680,10 @@ static int check_sizes(int64_t N, int64_t S, int64_t G, int64_t P, int metric) {
 }
 
 // permutations per batch: the metric matrix of one batch stays below 2 GiB
+// (or the tuned count)
 static int64_t batch_perms(int64_t N, int64_t P, size_t w) {
-    const int64_t cap = std::max<int64_t>(1, ((int64_t)2 << 30) / (N * (int64_t)w));
+    int64_t cap = std::max<int64_t>(1, ((int64_t)2 << 30) / (N * (int64_t)w));
+    if (g_tune.batch_perms > 0) cap = std::min(cap, g_tune.batch_perms);
     return std::min<int64_t>(P, cap);
 }
 
@@ -1186,7 +1105,22 @@ const char *gsea_metric_name(int id) {
 
 const char *gsea_last_error(void) { return g_last_error.c_str(); }
 
-const char *gsea_version(void) { return "cudagsea_amd 0.1.0 (gfx950)"; }
+const char *gsea_version(void) { return "cudagsea_amd 0.2.0 (gfx950)"; }
+
+int gsea_set_tuning(const gsea_tuning *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (t && t->batch_perms < 0)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "negative tuning value");
+    g_tune = t ? *t : gsea_tuning{};
+    return GSEA_OK;
+}
+
+int gsea_get_tuning(gsea_tuning *t) {
+    if (!t) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    *t = g_tune;
+    return GSEA_OK;
+}
 
 int gsea_device_count(int *count) {
     if (!count) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");

@@ -43,6 +43,20 @@ def _cstrings(strs):
     return arr, enc
 
 
+def set_tuning(**kw):
+    """Launch-shape tuning (gsea_set_tuning); no arguments = automatic."""
+    t = _lib.gsea_tuning()
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    check(lib().gsea_set_tuning(ctypes.byref(t)))
+
+
+def get_tuning():
+    t = _lib.gsea_tuning()
+    check(lib().gsea_get_tuning(ctypes.byref(t)))
+    return t.as_dict()
+
+
 # ---------------------------------------------------------------------------
 # engine calls on host arrays
 # ---------------------------------------------------------------------------

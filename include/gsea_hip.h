@@ -65,6 +65,20 @@ typedef struct gsea_timings {
     int64_t es_launches;  /* enrichment-score kernel launches timed */
 } gsea_timings;
 
+/* Launch-shape tuning (no counterpart in the reference, which fixes these
+ * choices: batch_scheduler.cuh:27-33, correlate_genes.cuh:162-163).  Zero
+ * fields mean "choose automatically".  The setting is process-wide, applies
+ * to the calls that follow, and never changes results: permutations are
+ * seeded by their global index, so any batch split gives the same bits.
+ *   batch_perms: permutations per batch (default: the metric rows of one
+ *                batch stay below 2 GiB) */
+typedef struct gsea_tuning {
+    int64_t batch_perms;
+    int64_t reserved[7];
+} gsea_tuning;
+int gsea_set_tuning(const gsea_tuning *tuning);   /* NULL: all automatic */
+int gsea_get_tuning(gsea_tuning *tuning);
+
 /* metric name <-> id; ids follow the dispatch order of
  * cudaGSEA/src/include/correlate_genes.cuh:166-295 */
 int         gsea_metric_id(const char *name);          /* -1 if unknown */

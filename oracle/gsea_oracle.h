@@ -74,6 +74,20 @@ void orc_scores_f64(const double *sorted, const uint32_t *index,
                     const int32_t *set_offsets, const int32_t *set_genes,
                     int64_t G, int64_t N, int64_t P, double *es);
 
+/* The same walks, also returning each walk's largest and smallest running
+ * sum (0 when the walk never goes above / below 0): es is whichever of the two
+ * has the larger magnitude, the first to occur on equal magnitudes.  Test
+ * infrastructure: where two implementations round these two candidates
+ * differently, a near-equal pair may flip the sign of es. */
+void orc_scores_ext_f32(const float *sorted, const uint32_t *index,
+                        const int32_t *set_offsets, const int32_t *set_genes,
+                        int64_t G, int64_t N, int64_t P, float *es,
+                        float *es_hi, float *es_lo);
+void orc_scores_ext_f64(const double *sorted, const uint32_t *index,
+                        const int32_t *set_offsets, const int32_t *set_genes,
+                        int64_t G, int64_t N, int64_t P, double *es,
+                        double *es_hi, double *es_lo);
+
 /* ---- statistics: statistics.cuh:46-117 (+ FDR, defined here) ----
  * es[G][P] is normalised in place (as the reference does); out[5][G] =
  * ES, NES, NP, FWER, FDR. */

@@ -70,6 +70,8 @@ def lib():
                 _P, _P, _I64, _I64, ctypes.c_int]
             getattr(L, "orc_scores_" + sfx).argtypes = [
                 _P, _P, _P, _P, _I64, _I64, _I64, _P]
+            getattr(L, "orc_scores_ext_" + sfx).argtypes = [
+                _P, _P, _P, _P, _I64, _I64, _I64, _P, _P, _P]
             getattr(L, "orc_final_statistics_" + sfx).argtypes = [
                 _P, _I64, _I64, _P]
             f = getattr(L, "orc_gsea_" + sfx)
@@ -169,6 +171,38 @@ def scores(sorted_keys, index, offsets, genes):
         _ptr(sorted_keys), _ptr(index), _ptr(offsets), _ptr(genes), G, N, P,
         _ptr(es))
     return es
+
+
+def scores_ext(sorted_keys, index, offsets, genes):
+    """(es, hi, lo) [G][P]: the scores and each walk's largest and smallest
+    running sum."""
+    sorted_keys = np.ascontiguousarray(sorted_keys)
+    double = sorted_keys.dtype == np.float64
+    P, N = sorted_keys.shape
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    genes = np.ascontiguousarray(genes, dtype=np.int32)
+    G = offsets.size - 1
+    es, hi, lo = (np.empty((G, P), dtype=sorted_keys.dtype) for _ in range(3))
+    getattr(lib(), "orc_scores_ext_" + ("f64" if double else "f32"))(
+        _ptr(sorted_keys), _ptr(index), _ptr(offsets), _ptr(genes), G, N, P,
+        _ptr(es), _ptr(hi), _ptr(lo))
+    return es, hi, lo
+
+
+def gsea_ext(X, labels, offsets, genes, P, metric, ascending=False, swap=False,
+             double=False):
+    """Whole CPU path with each walk's extremes: (stats, es, hi, lo)."""
+    dt = _dt(double)
+    X = np.ascontiguousarray(X, dtype=dt)
+    lab = np.ascontiguousarray(labels, dtype=np.uint8)
+    if swap:
+        lab = (lab + 1) % 2
+    r = correlate(X, lab, P, metric, sample_major=True, double=double)
+    sk, idx = pathway_sort(r, ascending=ascending)
+    es, hi, lo = scores_ext(sk, idx, offsets, genes)
+    st, _ = final_statistics(es.copy())
+    return st, es, hi, lo
 
 
 def final_statistics(es):
