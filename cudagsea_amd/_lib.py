@@ -87,6 +87,7 @@ SIGNATURES = {
                                         _I, _P, _P, _P, _P, _P, _P, _P]),
     "gsea_metric_f32": (_I, [_P, _I64, _I64, _P, _I64, _I64, _I, _I, _P, _P]),
     "gsea_metric_f64": (_I, [_P, _I64, _I64, _P, _I64, _I64, _I, _I, _P, _P]),
+    "gsea_metric_noalloc": (_I, [_I, _P, _I64, _I64, _P, _I64, _I64, _I, _I, _P, _P]),
     "gsea_rank_es_f32": (_I, [_P, _I64, _I64, _P, _P, _I64, _I, _P, _P]),
     "gsea_rank_es_f64": (_I, [_P, _I64, _I64, _P, _P, _I64, _I, _P, _P]),
     "gsea_argsort_f32": (_I, [_P, _I64, _I64, _I, _P, _P]),

@@ -236,26 +236,17 @@ static int metric_tile(int64_t S, bool *xlds) {
     return 64;
 }
 
+// k_metric over Pb permutations' class lists; rcp[i] = fl64(1/(i+1)), i < S
 template <class T>
-static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
-                         const uint16_t *lists, int64_t Pb, int metric,
-                         T *r, DeviceState *d, hipStream_t st) {
+static int launch_metric_rcp(const T *dX, int64_t N, int64_t S, const Labels &L,
+                             const uint16_t *lists, int64_t Pb, int metric, T *r,
+                             const double *rcp, const DeviceState *d, hipStream_t st) {
     MetricArgs<T> a;
     a.X = dX; a.N = (int)N; a.S = (int)S; a.lists = lists;
     a.lstride = L.lstride; a.nA = L.nA; a.nB = L.nB; a.offB = L.offB;
     a.lenA = L.lenA;
     a.Pb = (int)Pb; a.metric = metric; a.r = r;
-    // 1/(i+1) for the Welford running counts, correctly rounded doubles
-    if (d->rcp_len < S) {
-        std::vector<double> rcp((size_t)S);
-        for (int64_t i = 0; i < S; ++i) rcp[(size_t)i] = 1.0 / (double)(i + 1);
-        void *pr;
-        int rc = ensure(d, kBufRcp, sizeof(double) * S, &pr);
-        if (rc) return rc;
-        HIP_TRY(hipMemcpy(pr, rcp.data(), sizeof(double) * S, hipMemcpyHostToDevice));
-        d->rcp_len = S;
-    }
-    a.rcp = (const double *)d->bufs[kBufRcp].ptr;
+    a.rcp = rcp;
     bool xlds = false;
     const int gt = metric_tile<T>(S, &xlds);
     const int64_t gx = (N + gt - 1) / gt;
@@ -267,6 +258,24 @@ static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
     a.ppb = (int)ppb;
     HIP_TRY(launch_metric_kernel<T>(a, acc_kind(metric), gt, xlds, st));
     return GSEA_OK;
+}
+
+template <class T>
+static int launch_metric(const T *dX, int64_t N, int64_t S, const Labels &L,
+                         const uint16_t *lists, int64_t Pb, int metric,
+                         T *r, DeviceState *d, hipStream_t st) {
+    // 1/(i+1) for the Welford running counts, correctly rounded doubles
+    if (d->rcp_len < S) {
+        std::vector<double> rcp((size_t)S);
+        for (int64_t i = 0; i < S; ++i) rcp[(size_t)i] = 1.0 / (double)(i + 1);
+        void *pr;
+        int rc = ensure(d, kBufRcp, sizeof(double) * S, &pr);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(pr, rcp.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+        d->rcp_len = S;
+    }
+    return launch_metric_rcp<T>(dX, N, S, L, lists, Pb, metric, r,
+                                (const double *)d->bufs[kBufRcp].ptr, d, st);
 }
 
 // LDS layouts.  k_rank: [keys (fp32 only)] | ids | 16 x 256 counters | misc.
@@ -1049,6 +1058,125 @@ static int metric_stage(const T *dX, int64_t N, int64_t S, const uint8_t *labels
     return GSEA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free metric stage.  Its scratch -- the labels, the permuted
+// labels and class lists of every permutation, and the Welford reciprocal
+// table -- lives in `scratch`, a static device array that every GPU the
+// library runs on holds until the process ends.  Host code reaches it only
+// through the per-device address hipGetSymbolAddress returns (the array's
+// name is a host-side stand-in, not a device address).  All calls on one GPU
+// share it: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static void *g_scratch[64];   // per device: device address of `scratch`
+
+static int scratch_ptr(int dev, void **out) {
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GSEA_ERR_HIP, "no device address for the static scratch");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return GSEA_OK;
+}
+
+// labels (0/1 bytes) swapped in place
+__global__ void k_swap_labels(uint8_t *lab, int S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) lab[i] = (uint8_t)((lab[i] + 1) & 1);
+}
+
+// rcp[i] = 1/(i+1), correctly rounded (IEEE double division)
+__global__ void k_recip_table(double *rcp, int S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) rcp[i] = 1.0 / (double)(i + 1);
+}
+
+// class sizes exactly as prepare_labels counts them, without a host copy
+static int label_counts(const uint8_t *labels, int64_t S, int flags, Labels *L) {
+    int n0 = 0, n1 = 0, odd = 0;
+    for (int64_t i = 0; i < S; ++i) {
+        n0 += labels[i] == 0;
+        n1 += labels[i] == 1;
+        odd += labels[i] & 1;
+    }
+    const bool swap = (flags & GSEA_FLAG_SWAP_LABELS) != 0;
+    L->nA = swap ? n1 : n0;
+    L->nB = swap ? n0 : n1;
+    if (L->nA == 0 || L->nB == 0)
+        return fail(GSEA_ERR_INVALID_ARGUMENT,
+                    "both phenotype classes need at least one sample");
+    // label-0 samples after the swap (v -> (v + 1) & 1 maps odd v to 0)
+    L->lenA = swap ? odd : n0;
+    L->offB = (L->lenA + 7) / 8 * 8;
+    L->lstride = L->offB + ((int)S - L->lenA + 7) / 8 * 8;
+    return GSEA_OK;
+}
+
+// bytes of scratch a metric call of P permutations over S samples needs
+static size_t noalloc_layout(int64_t S, int64_t P, int lstride, size_t off[4]) {
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    off[0] = 0;                                        // labels [S]
+    off[1] = off[0] + up((size_t)S);                   // rcp [S] doubles
+    off[2] = off[1] + up(sizeof(double) * (size_t)S);  // permuted labels [P][S]
+    off[3] = off[2] + up((size_t)P * (size_t)S);       // class lists [P][lstride]
+    return off[3] + (size_t)P * (size_t)lstride * 2;
+}
+
+template <class T>
+static int metric_stage_noalloc(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                                int64_t P, int64_t shift, int metric, int flags, T *d_r,
+                                hipStream_t st) {
+    int rc = check_sizes(N, S, 1, P, metric);
+    if (rc) return rc;
+    if (shift < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "negative permutation shift");
+    if (!dX || !labels || !d_r) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    Labels L;
+    rc = label_counts(labels, S, flags, &L);
+    if (rc) return rc;
+    size_t off[4];
+    const size_t need = noalloc_layout(S, P, L.lstride, off);
+    if (need > kNoAllocScratchBytes)
+        return fail(GSEA_ERR_UNSUPPORTED_SIZE,
+                    "allocation-free metric call needs " + std::to_string(need) +
+                        " bytes of scratch, more than its " +
+                        std::to_string(kNoAllocScratchBytes) + " (fewer permutations per call)");
+    DeviceState *d = nullptr;
+    rc = device_state(&d);
+    if (rc) return rc;
+    int dev = 0;
+    if ((rc = current_device(&dev))) return rc;
+    void *base = nullptr;
+    if ((rc = scratch_ptr(dev, &base))) return rc;
+    unsigned char *sp = (unsigned char *)base;
+    uint8_t *plab = sp + off[0];
+    double *prcp = (double *)(sp + off[1]);
+    uint8_t *psig = sp + off[2];
+    uint16_t *plists = (uint16_t *)(sp + off[3]);
+    // every scratch byte a kernel reads is written first in this call: the
+    // labels and reciprocals here, permuted labels and class lists by
+    // k_class_lists (k_metric reads list entries [0, nA) and
+    // [offB, offB + S - nA) only)
+    HIP_TRY(hipMemcpyAsync(plab, labels, (size_t)S, hipMemcpyHostToDevice, st));
+    const unsigned sb = (unsigned)((S + 255) / 256);
+    if (flags & GSEA_FLAG_SWAP_LABELS) {
+        hipLaunchKernelGGL(k_swap_labels, dim3(sb), dim3(256), 0, st, plab, (int)S);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_recip_table, dim3(sb), dim3(256), 0, st, prcp, (int)S);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, st,
+                       (const uint8_t *)plab, (int)S, shift, (int)P, psig, L.offB, plists,
+                       L.lstride);
+    HIP_TRY(hipGetLastError());
+    rc = launch_metric_rcp<T>(dX, N, S, L, plists, P, metric, d_r, prcp, d, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
 template <class T>
 static int rank_es_stage(const T *d_r, int64_t N, int64_t P, const int32_t *d_off,
                          const int32_t *d_genes, int64_t G, int flags, T *d_es,
@@ -1226,6 +1354,17 @@ int gsea_metric_f64(const double *dX, int64_t N, int64_t S, const uint8_t *label
     std::lock_guard<std::mutex> lk(g_mu);
     return metric_stage<double>(dX, N, S, labels, P, shift, metric, flags, d_r,
                                 (hipStream_t)stream);
+}
+
+int gsea_metric_noalloc(int double_precision, const void *d_exprs, int64_t N, int64_t S,
+                        const uint8_t *labels, int64_t P, int64_t shift, int metric,
+                        int flags, void *d_r, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (double_precision)
+        return metric_stage_noalloc<double>((const double *)d_exprs, N, S, labels, P, shift,
+                                            metric, flags, (double *)d_r, (hipStream_t)stream);
+    return metric_stage_noalloc<float>((const float *)d_exprs, N, S, labels, P, shift, metric,
+                                       flags, (float *)d_r, (hipStream_t)stream);
 }
 
 int gsea_rank_es_f32(const float *d_r, int64_t N, int64_t P, const int32_t *d_off,

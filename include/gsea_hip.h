@@ -183,6 +183,23 @@ int gsea_metric_f64(const double *d_exprs, int64_t num_genes,
                     int64_t num_samples, const uint8_t *labels,
                     int64_t num_perms, int64_t shift, int metric_id, int flags,
                     double *d_r, void *stream);
+/* gsea_metric_f32/f64 without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (labels, each permutation's permuted labels and class lists, the
+ * Welford reciprocal table) lives in a static device array of 32 MiB that
+ * each GPU the library runs on holds until the process ends; a call needs
+ * at most num_perms * (3 * num_samples + 28) + 9 * num_samples + 1024 bytes
+ * of it and larger calls are
+ * rejected with GSEA_ERR_UNSUPPORTED_SIZE (split the permutations with
+ * `shift`).  Calls on one GPU share that array: calls that overlap on one GPU
+ * race.  d_exprs and d_r are float (double_precision == 0) or double.
+ * Results equal gsea_metric_f32/f64's. */
+int gsea_metric_noalloc(int double_precision, const void *d_exprs, int64_t num_genes,
+                        int64_t num_samples, const uint8_t *labels, int64_t num_perms,
+                        int64_t shift, int metric_id, int flags, void *d_r,
+                        void *stream);
+
 int gsea_rank_es_f32(const float *d_r, int64_t num_genes, int64_t num_perms,
                      const int32_t *d_set_offsets, const int32_t *d_set_genes,
                      int64_t num_sets, int flags, float *d_es, void *stream);
