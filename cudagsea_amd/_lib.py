@@ -37,10 +37,15 @@ class gsea_extension(ctypes.Structure):
 
 
 class gsea_tuning(ctypes.Structure):
-    _fields_ = [("batch_perms", ctypes.c_int64), ("reserved", ctypes.c_int64 * 7)]
+    _fields_ = [("batch_perms", ctypes.c_int64), ("metric_gene_tile", ctypes.c_int32),
+                ("metric_perms_per_block", ctypes.c_int32), ("es_waves", ctypes.c_int32),
+                ("es_lane_set_max", ctypes.c_int32), ("reserved32", ctypes.c_int32),
+                ("reserved", ctypes.c_int64 * 5)]
+    KNOBS = ("batch_perms", "metric_gene_tile", "metric_perms_per_block", "es_waves",
+             "es_lane_set_max")
 
     def as_dict(self):
-        return {"batch_perms": self.batch_perms}
+        return {k: getattr(self, k) for k in self.KNOBS}
 
 
 class gsea_timings(ctypes.Structure):

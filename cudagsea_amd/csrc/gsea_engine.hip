@@ -225,10 +225,17 @@ static int prepare_labels(const uint8_t *labels, int64_t S, int flags,
     return GSEA_OK;
 }
 
-// X tile width: the widest of 64/32/16 genes whose [S][GT] tile fits 64 KiB
-// (two workgroups per CU); otherwise X is read from global memory.
+// X tile width: the tuned width if its [S][GT] tile fits LDS, else the
+// widest of 64/32/16 genes whose tile fits 64 KiB (two workgroups per CU);
+// otherwise X is read from global memory.
 template <class T>
-static int metric_tile(int64_t S, bool *xlds) {
+static int metric_tile(int64_t S, bool *xlds, int lds_max) {
+    const int tuned = g_tune.metric_gene_tile;
+    if ((tuned == 16 || tuned == 32 || tuned == 64) &&
+        (size_t)S * tuned * sizeof(T) <= (size_t)lds_max) {
+        *xlds = true;
+        return tuned;
+    }
     for (int gt : {64, 32, 16}) {
         if ((size_t)S * gt * sizeof(T) <= 65536) { *xlds = true; return gt; }
     }
@@ -248,14 +255,15 @@ static int launch_metric_rcp(const T *dX, int64_t N, int64_t S, const Labels &L,
     a.Pb = (int)Pb; a.metric = metric; a.r = r;
     a.rcp = rcp;
     bool xlds = false;
-    const int gt = metric_tile<T>(S, &xlds);
+    const int gt = metric_tile<T>(S, &xlds, d->lds_max);
     const int64_t gx = (N + gt - 1) / gt;
     // enough workgroups to fill the device several times over, each streaming
     // as many permutations as possible over its X tile
     const int64_t target = 8LL * d->cu_count;
     int64_t ppb = (Pb * gx + target - 1) / target;
     ppb = std::max<int64_t>(32, std::min<int64_t>(512, (ppb + 31) / 32 * 32));
-    a.ppb = (int)ppb;
+    if (g_tune.metric_perms_per_block > 0) ppb = g_tune.metric_perms_per_block;
+    a.ppb = (int)std::min<int64_t>(ppb, std::max<int64_t>(Pb, 1));
     HIP_TRY(launch_metric_kernel<T>(a, acc_kind(metric), gt, xlds, st));
     return GSEA_OK;
 }
@@ -325,8 +333,9 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     // k_es: |metric| by rank and pos for genes/ranks 0..N (the sentinel
     // included), not padded to the rank kernel's chunks
     const size_t nes = (size_t)N + 1;
+    const int ew_max = g_tune.es_waves > 0 ? std::min(g_tune.es_waves, kRankWaves) : kRankWaves;
     for (int alds = (sizeof(T) == 4) ? 1 : 0; alds >= 0; --alds) {
-        for (int ew = kRankWaves; ew >= 1; --ew) {
+        for (int ew = ew_max; ew >= 1; --ew) {
             if (fit(alds ? nes * 4 : 0, nes, (size_t)ew * pl->nw * 4, d->lds_max, &pl->es)) {
                 pl->es.alds = alds != 0;
                 pl->es.waves = ew;
@@ -345,7 +354,7 @@ struct SetTiers {
     int chunk_end64 = 0;
     int stride = 0;
     const int4 *large_desc = nullptr;
-    const int32_t *small_ids = nullptr, *small_k = nullptr;
+    const int32_t *small_ids = nullptr, *small_k = nullptr, *small_kmax = nullptr;
     const uint16_t *small_gm = nullptr;
     const uint16_t *large_genes = nullptr;   // large sets' gene ids + 1
     const void *decay = nullptr;         // [G] of T
@@ -368,17 +377,26 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
         for (int64_t s = 0; s < G; ++s) all[(size_t)s] = (int32_t)s;
         subset = &all;
     }
+    // sets up to lane_max genes: one lane each (tuned: 8..128, or none)
+    const int lane_max = g_tune.es_lane_set_max < 0 ? 0
+                       : g_tune.es_lane_set_max > 0 ? std::min(g_tune.es_lane_set_max, kSmallSet)
+                       : kSmallSet;
     std::vector<int32_t> large, cls[5];
     for (int32_t s : *subset) {
         const int k = off[(size_t)s + 1] - off[(size_t)s];
         if (k < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
-        if (k > kSmallSet) large.push_back((int32_t)s);
+        if (k > lane_max) large.push_back((int32_t)s);
         else cls[k <= 8 ? 0 : k <= 16 ? 1 : k <= 32 ? 2 : k <= 64 ? 3 : 4].push_back((int32_t)s);
     }
     // large sets stay in set order: a workgroup's wavefronts claim items in
     // order, so the ES stores of one permutation row fill cache lines
     // together (with thousands of items per permutation the tail imbalance
     // of this order is a few microseconds)
+    // within a class, sets of similar size share a chunk of 64 lanes
+    for (auto &c : cls)
+        std::stable_sort(c.begin(), c.end(), [&](int32_t x, int32_t y) {
+            return off[x + 1] - off[x] < off[y + 1] - off[y];
+        });
     std::vector<int32_t> sid, sk;
     int ends[5];
     for (int c = 0; c < 5; ++c) {
@@ -398,7 +416,8 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     // small sizes.  offset: the set's first entry in this list's 16-bit copy
     // of the large sets' gene ids (made on every call: the caller's CSR may
     // change between calls at the same address)
-    std::vector<int32_t> host(4 * nl + 2 * ns);
+    const size_t nch = ns / kWave;
+    std::vector<int32_t> host(4 * nl + 2 * ns + nch);
     int64_t nlg = 0;
     for (size_t i = 0; i < nl; ++i) {
         const int32_t s = large[i];
@@ -410,6 +429,10 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     }
     std::copy(sid.begin(), sid.end(), host.begin() + 4 * nl);
     std::copy(sk.begin(), sk.end(), host.begin() + 4 * nl + ns);
+    // the largest set of each chunk bounds its lanes' gather and walk
+    for (size_t c = 0; c < nch; ++c)
+        host[4 * nl + 2 * ns + c] =
+            *std::max_element(sk.begin() + c * kWave, sk.begin() + (c + 1) * kWave);
     // per-set decay of the running sum, enrichment_scores.cuh:311, by set id
     // and again in large-item order
     std::vector<T> decay((size_t)G + nl);
@@ -436,6 +459,7 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     t->large_desc = (const int4 *)pt;
     t->small_ids = (const int32_t *)pt + 4 * nl;
     t->small_k = (const int32_t *)pt + 4 * nl + ns;
+    t->small_kmax = (const int32_t *)pt + 4 * nl + 2 * ns;
     t->small_gm = (const uint16_t *)pg;
     t->large_genes = (const uint16_t *)pl;
     if (nl) {
@@ -475,6 +499,7 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.small_ids = tiers ? tiers->small_ids : nullptr;
     a.small_k = tiers ? tiers->small_k : nullptr;
     a.small_gm = tiers ? tiers->small_gm : nullptr;
+    a.small_kmax = tiers ? tiers->small_kmax : nullptr;
     a.small_stride = tiers ? tiers->stride : 0;
     a.chunk_end8 = tiers ? tiers->chunk_end8 : 0;
     a.chunk_end16 = tiers ? tiers->chunk_end16 : 0;
@@ -1237,8 +1262,13 @@ const char *gsea_version(void) { return "cudagsea_amd 0.2.0 (gfx950)"; }
 
 int gsea_set_tuning(const gsea_tuning *t) {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (t && t->batch_perms < 0)
-        return fail(GSEA_ERR_INVALID_ARGUMENT, "negative tuning value");
+    if (t) {
+        const int gt = t->metric_gene_tile, lm = t->es_lane_set_max;
+        if (t->batch_perms < 0 || t->metric_perms_per_block < 0 || t->es_waves < 0 ||
+            t->es_waves > kRankWaves || (gt && gt != 16 && gt != 32 && gt != 64) ||
+            (lm > 0 && lm != 8 && lm != 16 && lm != 32 && lm != 64 && lm != 128) || lm < -1)
+            return fail(GSEA_ERR_INVALID_ARGUMENT, "tuning value out of range");
+    }
     g_tune = t ? *t : gsea_tuning{};
     return GSEA_OK;
 }

@@ -61,6 +61,7 @@ struct RankArgs {
     const int32_t *small_ids;          // [n_chunks*64] set id or -1
     const int32_t *small_k;            // [n_chunks*64] set sizes
     const uint16_t *small_gm;          // [kSmallSet][small_stride] gene ids + 1
+    const int32_t *small_kmax;         // [n_chunks] largest set of each chunk
     int small_stride;
     int chunk_end8, chunk_end16, chunk_end32, chunk_end64;   // chunk class boundaries
     const T *decay;                    // [G] (T)(-1.0 / (double)(N - k_s))
@@ -142,6 +143,32 @@ struct HitShare {
     }
 };
 
+// min / max of two doubles with NaN operands skipped (IEEE-mode v_min_f64 /
+// v_max_f64).  The walk's accumulators only ever hold quiet NaN or results
+// of these instructions, so the canonicalisation the compiler adds around
+// fmin / fmax (for signalling NaN inputs) is not needed.
+__device__ __forceinline__ double min_skip_nan(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double max_skip_nan(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// the same for float (operands are quiet NaN or arithmetic results)
+__device__ __forceinline__ float min_skip_nan(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float max_skip_nan(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // leftmost lane value of largest |value| (first strict maximum in lane
 // order), returned on every lane.  Lane values here are never NaN or -0.
 template <class T>
@@ -218,7 +245,15 @@ __device__ __forceinline__ void bitonic_sort_u16x2(uint32_t (&v)[NC / 2]) {
 // Sets of at most NC genes: one lane per set.  The lane gathers its hit
 // ranks, sorts them in registers and walks them in rank order.  Rows t >= k
 // of the gathered gene ids hold the sentinel: rank N (sorted last, as
-// N <= 65535) and |metric| 0 (nothing added to N_R); the walk masks them.
+// N <= 65535) and |metric| 0 (nothing added to N_R).  The sets of a chunk are
+// of similar size (sorted by size within their class) and kmax, the chunk's
+// largest, bounds the gather and the walk: groups of 8 entries at or past it
+// are sentinels only, and their keys are the sentinel from the start.
+// The walk keeps the extremes of the running sum just before and just after
+// each hit (in double, c unrounded, as the wavefront path); rounding to T is
+// monotone, so the enrichment score is the rounded extreme of larger
+// magnitude, and only on equal magnitudes (the reference keeps the first)
+// does the lane walk again with the first-strict-maximum rule.
 template <class T, int NC, class AF>
 __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
                                               const uint16_t *pos, AF &A,
@@ -227,7 +262,11 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     const int slot = chunk * kWave + lane;
     const int sid = a.small_ids[slot];
     const int k = a.small_k[slot];
+    const int kmax = __builtin_amdgcn_readfirstlane(a.small_kmax[chunk]);
     uint32_t v[R];
+    const uint32_t sent = (uint32_t)a.N * 0x00010001u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = sent;
     const uint16_t *gcol = a.small_gm + slot;
     // opaque per call: keeps the compiler from hoisting the row offsets out
     // of the work loop into (spilled) scalar registers
@@ -239,6 +278,7 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     double nr = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
+        if (t0 >= kmax) continue;   // uniform: keeps the loop fully unrolled
         int gg[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
@@ -251,8 +291,8 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int t = t0 + u;
-            if (t < R) v[t] = g[u];
-            else v[t - R] |= g[u] << 16;
+            if (t < R) v[t] = (v[t] & 0xffff0000u) | g[u];
+            else v[t - R] = (v[t - R] & 0xffffu) | (g[u] << 16);
             nr += (double)x[u];
         }
         // the group's results are final here: its loads, masks and values
@@ -272,10 +312,13 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
 
     const double decay = sid >= 0 ? (double)a.decay[sid] : 0.0;
-    T best = T(0);
-    double Sa = 0.0;
+    // k == N: no misses; the reference's decay (-inf) is never applied
+    const double dec = k == a.N ? 0.0 : decay;
+    const double qnan = __builtin_nan("");
+    double L = 0.0, vmin = qnan, vmax = qnan;
 #pragma unroll
     for (int t0 = 0; t0 < NC; t0 += 8) {
+        if (t0 >= kmax) continue;   // uniform: keeps the loop fully unrolled
         T x[8];
         // an opaque copy of k per group: the compiler would otherwise compute
         // all NC masks t < k at once and keep them live (spills at NC = 128)
@@ -284,10 +327,34 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
 #pragma unroll
         for (int u = 0; u < 8; ++u) x[u] = A(rank_at(t0 + u));
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            es_candidates<T>(Sa, share.rounded(x[u]), rank_at(t0 + u), t0 + u, decay,
-                             best, t0 + u < kk);
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u;
+            const bool valid = t < kk;
+            const double before = __builtin_fma((double)(rank_at(t) - t), dec, L);
+            vmin = min_skip_nan(vmin, valid ? before : qnan);
+            const double c = share(x[u]);   // 0 for a sentinel
+            L += c;
+            vmax = max_skip_nan(vmax, valid ? before + c : qnan);
+        }
         __builtin_amdgcn_sched_barrier(0);
+    }
+    const T hi = (T)vmax, lo = (T)vmin;
+    const T ah = hi == hi ? fabs(hi) : T(0);
+    const T al = lo == lo ? fabs(lo) : T(0);
+    T best = ah > al ? hi : (al > ah ? lo : T(0));
+    if (ah == al && ah != T(0)) {
+        // equal magnitudes: the first strictly largest candidate, in order
+        best = T(0);
+        double Sa = 0.0;
+#pragma unroll
+        for (int t0 = 0; t0 < NC; t0 += 8) {
+            if (t0 >= kmax) continue;   // uniform: keeps the loop fully unrolled
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u;
+                es_candidates<T>(Sa, share(A(rank_at(t))), rank_at(t), t, decay, best, t < k);
+            }
+        }
     }
     if (k == 0) best = (T)((double)a.N * decay);
     if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
@@ -323,7 +390,7 @@ __device__ __forceinline__ void es_mark_hits(__amdgpu_buffer_rsrc_t rs, const ui
 }
 
 // All of a set's entries through es_mark_hits, 8 slots per lane per round;
-// the last round takes the smallest batch that covers the set's end.  The
+// the last round takes exactly the slots that hold set entries.  The
 // sentinel's bit (rank N) is cleared at the end.  sg: the set's ids + 1.
 template <class T, class AF>
 __device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *pos,
@@ -335,10 +402,16 @@ __device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *
         __builtin_amdgcn_make_buffer_rsrc((void *)sg, (short)0, 2 * k, kBufferRsrcWord3);
     for (int j0 = 0; j0 < k; j0 += 8 * kWave) {
         const int ns = (k - j0 + kWave - 1) / kWave;   // slots with set entries
-        if (ns > 4) es_mark_hits<T, 8>(rs, pos, A, bm, j0, lane, nr);
-        else if (ns > 2) es_mark_hits<T, 4>(rs, pos, A, bm, j0, lane, nr);
-        else if (ns > 1) es_mark_hits<T, 2>(rs, pos, A, bm, j0, lane, nr);
-        else es_mark_hits<T, 1>(rs, pos, A, bm, j0, lane, nr);
+        switch (ns < 8 ? ns : 8) {
+        case 8: es_mark_hits<T, 8>(rs, pos, A, bm, j0, lane, nr); break;
+        case 7: es_mark_hits<T, 7>(rs, pos, A, bm, j0, lane, nr); break;
+        case 6: es_mark_hits<T, 6>(rs, pos, A, bm, j0, lane, nr); break;
+        case 5: es_mark_hits<T, 5>(rs, pos, A, bm, j0, lane, nr); break;
+        case 4: es_mark_hits<T, 4>(rs, pos, A, bm, j0, lane, nr); break;
+        case 3: es_mark_hits<T, 3>(rs, pos, A, bm, j0, lane, nr); break;
+        case 2: es_mark_hits<T, 2>(rs, pos, A, bm, j0, lane, nr); break;
+        default: es_mark_hits<T, 1>(rs, pos, A, bm, j0, lane, nr); break;
+        }
     }
     // LDS operations of one wavefront complete in order: this follows the ORs
     if (lane == 0) atomicAnd(&bm[N >> 5], ~(1u << (N & 31)));
@@ -352,32 +425,6 @@ __device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *
 // fmin / fmax skip it, as the reference's comparison skips a NaN running
 // sum).  EXACT == true: L starts at the lane's Sa and every candidate goes
 // through es_candidates (first strict maximum, rounded to T).
-// min / max of two doubles with NaN operands skipped (IEEE-mode v_min_f64 /
-// v_max_f64).  The walk's accumulators only ever hold quiet NaN or results
-// of these instructions, so the canonicalisation the compiler adds around
-// fmin / fmax (for signalling NaN inputs) is not needed.
-__device__ __forceinline__ double min_skip_nan(double a, double b) {
-    double r;
-    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ double max_skip_nan(double a, double b) {
-    double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-// the same for float (operands are quiet NaN or arithmetic results)
-__device__ __forceinline__ float min_skip_nan(float a, float b) {
-    float r;
-    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-__device__ __forceinline__ float max_skip_nan(float a, float b) {
-    float r;
-    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-
 template <class T, bool EXACT, class AF>
 __device__ __forceinline__ void es_walk(uint32_t *bm, int W, AF &A,
                                         const HitShare<T> &share, int lane,
@@ -679,7 +726,7 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                     const T dec = *((const __attribute__((address_space(4))) T *)
                                         a.large_decay + item);
                     es_large_set<T, ALDS>(a, pos, A, bm, gp, d[0], d[1], d[2], (double)dec,
-                                             lane);
+                                          lane);
                 } else {
                     const int chunk = item - a.n_large;
                     if (chunk < a.chunk_end8)

@@ -66,15 +66,30 @@ typedef struct gsea_timings {
 } gsea_timings;
 
 /* Launch-shape tuning (no counterpart in the reference, which fixes these
- * choices: batch_scheduler.cuh:27-33, correlate_genes.cuh:162-163).  Zero
- * fields mean "choose automatically".  The setting is process-wide, applies
- * to the calls that follow, and never changes results: permutations are
- * seeded by their global index, so any batch split gives the same bits.
- *   batch_perms: permutations per batch (default: the metric rows of one
- *                batch stay below 2 GiB) */
+ * choices: batch_scheduler.cuh:27-33, correlate_genes.cuh:162-163,
+ * batched_sort.cuh:85-90, enrichment_scores.cuh:240-245).  Zero fields mean
+ * "choose automatically".  The setting is process-wide, applies to the calls
+ * that follow, and never changes results: permutations are seeded by their
+ * global index and every shape computes the same arithmetic.
+ *   batch_perms        permutations per batch (default: the metric rows of
+ *                      one batch stay below 2 GiB)
+ *   metric_gene_tile   genes per k_metric workgroup: 16, 32 or 64 (the X
+ *                      tile [S][tile] must fit LDS; default: the widest
+ *                      within 64 KiB)
+ *   metric_perms_per_block  permutations a k_metric workgroup streams over
+ *                      its X tile (default: enough workgroups for 8 per CU)
+ *   es_waves           wavefronts of each k_es workgroup that score sets,
+ *                      1..16 (default: as many as the LDS holds)
+ *   es_lane_set_max    largest set scored by one lane (8, 16, 32, 64 or 128;
+ *                      -1: every set by a wavefront; default 128) */
 typedef struct gsea_tuning {
     int64_t batch_perms;
-    int64_t reserved[7];
+    int32_t metric_gene_tile;
+    int32_t metric_perms_per_block;
+    int32_t es_waves;
+    int32_t es_lane_set_max;
+    int32_t reserved32;
+    int64_t reserved[5];
 } gsea_tuning;
 int gsea_set_tuning(const gsea_tuning *tuning);   /* NULL: all automatic */
 int gsea_get_tuning(gsea_tuning *tuning);
