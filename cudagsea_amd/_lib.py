@@ -86,6 +86,10 @@ SIGNATURES = {
                                  _I, _P, _P, _P, _P]),
     "gsea_run_device_f64": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,
                                  _I, _P, _P, _P, _P]),
+    "gsea_sweep_device_f32": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _P, _I, _I, _I,
+                                   _P, _P, _P]),
+    "gsea_sweep_device_f64": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _P, _I, _I, _I,
+                                   _P, _P, _P]),
     "gsea_run_budget_device_f32": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,
                                         _I, _P, _P, _P, _P, _P, _P, _P]),
     "gsea_run_budget_device_f64": (_I, [_P, _I64, _I64, _P, _P, _P, _I64, _I64, _I,

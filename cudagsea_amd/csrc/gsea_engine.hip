@@ -68,12 +68,22 @@ enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
     kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp,
-    kBufExtSorted, kBufFdrTable, kNumBufs
+    kBufFdrTable, kBufFdrInfo, kBufCheck, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
 constexpr int kTierSlots = 24;
 enum TierBuf { kTierIds, kTierGm, kTierDecay, kTierGenes, kNumTierBufs };
+
+// the set tiers of the last set list (slot 0), reused while the list's
+// pointers, sizes, split and contents (checksum) are unchanged
+struct TierKey {
+    const int32_t *off = nullptr, *genes = nullptr;
+    int64_t G = -1, N = -1;
+    int lane_max = -1;
+    size_t tsize = 0;
+    unsigned long long sum = 0, total = 0;
+};
 
 struct DeviceState {
     bool init = false;
@@ -82,6 +92,9 @@ struct DeviceState {
     Buffer bufs[kNumBufs];
     Buffer tier_bufs[kTierSlots][kNumTierBufs];
     int64_t rcp_len = 0;   // entries of the 1/(i+1) table in kBufRcp
+    TierKey tier_key;
+    bool tier_valid = false;
+    hipStream_t own_stream = nullptr;   // capture stream for callers on the null stream
     std::vector<hipEvent_t> events;
 };
 
@@ -118,9 +131,13 @@ static int device_state(DeviceState **out) {
     return GSEA_OK;
 }
 
+// bumped whenever a workspace buffer moves: captured graphs hold addresses
+static uint64_t g_ws_generation = 0;
+
 static int ensure_buf(Buffer &b, size_t bytes, void **out) {
     bytes = std::max<size_t>(bytes, 256);
     if (b.cap < bytes) {
+        ++g_ws_generation;
         if (b.ptr) HIP_TRY(hipFree(b.ptr));
         b.ptr = nullptr;
         b.cap = 0;
@@ -441,6 +458,7 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     for (size_t i = 0; i < nl; ++i) decay[(size_t)G + i] = decay[(size_t)large[i]];
     void *pt, *pg, *pd, *pl;
     Buffer *tb = d->tier_bufs[slot];
+    if (slot == 0) d->tier_valid = false;   // slot 0's buffers change below
     int rc = ensure_buf(tb[kTierIds], sizeof(int32_t) * std::max<size_t>(host.size(), 1), &pt);
     // 1 KiB of slack after the last set: a phase-A load past a set's end is
     // range-checked against that set, never outside the allocation
@@ -474,6 +492,47 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     }
     // the host vectors above must outlive the asynchronous copy
     HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
+// slot-0 tiers of the last set list, per device
+static SetTiers g_tier0[64];
+
+// tiers of every set of a list, reused across calls while the list is the
+// same (pointers, sizes, lane split, value type and contents by checksum)
+template <class T>
+static int cached_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G, int64_t N,
+                        DeviceState *d, hipStream_t st, SetTiers *out) {
+    int dev = 0;
+    int rc = current_device(&dev);
+    if (rc) return rc;
+    void *pc;
+    rc = ensure(d, kBufCheck, 2 * sizeof(unsigned long long), &pc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_csr_checksum, dim3(1), dim3(1024), 0, st, d_off, d_genes, G,
+                       (unsigned long long *)pc);
+    HIP_TRY(hipGetLastError());
+    unsigned long long sum[2];
+    HIP_TRY(hipMemcpyAsync(sum, pc, sizeof(sum), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    TierKey k;
+    k.off = d_off; k.genes = d_genes; k.G = G; k.N = N;
+    k.lane_max = g_tune.es_lane_set_max;
+    k.tsize = sizeof(T);
+    k.sum = sum[0];
+    k.total = sum[1];
+    const TierKey &c = d->tier_key;
+    if (d->tier_valid && c.off == k.off && c.genes == k.genes && c.G == k.G && c.N == k.N &&
+        c.lane_max == k.lane_max && c.tsize == k.tsize && c.sum == k.sum && c.total == k.total) {
+        *out = g_tier0[dev];
+        return GSEA_OK;
+    }
+    d->tier_valid = false;
+    rc = build_tiers<T>(d_off, d_genes, G, N, d, st, out);
+    if (rc) return rc;
+    g_tier0[dev] = *out;
+    d->tier_key = k;
+    d->tier_valid = true;
     return GSEA_OK;
 }
 
@@ -577,120 +636,184 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
 // statistics
 // ---------------------------------------------------------------------------
 template <class T>
-static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
-                     DeviceState *d, hipStream_t st,
-                     const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
-    void *pf, *pmax, *pmin;
+struct StatsBufs {
+    T *factor = nullptr, *maxv = nullptr, *minv = nullptr, *thr = nullptr;
+    unsigned long long *hist = nullptr;   // hub | hlb | signs[2] | hmax | hmin
+    int32_t *table = nullptr;
+    FdrInfo<T> *info = nullptr;
+    int npow2 = 0;        // keys of the device threshold sort (0: host sort)
+    size_t fdr_lds = 0;   // k_fdr_hist's LDS (0: histograms in global memory)
+};
+
+static int64_t hist_len(int64_t G) { return 4 * (G + 1) + 2; }
+
+// every buffer the statistics of P x G scores use, and the kernels' LDS
+// limits (before any capture: nothing here is a stream operation)
+template <class T>
+static int prepare_stats(int64_t P, int64_t G, DeviceState *d, StatsBufs<T> *b) {
+    void *pf, *pmax, *pmin, *pthr, *phist, *ptab, *pinfo;
     int rc = ensure(d, kBufFactor, (size_t)G * sizeof(T), &pf);
     if (!rc) rc = ensure(d, kBufMax, (size_t)P * sizeof(T), &pmax);
     if (!rc) rc = ensure(d, kBufMin, (size_t)P * sizeof(T), &pmin);
+    if (!rc) rc = ensure(d, kBufThr, (size_t)G * sizeof(T), &pthr);
+    if (!rc) rc = ensure(d, kBufHist, sizeof(unsigned long long) * hist_len(G), &phist);
+    if (!rc) rc = ensure(d, kBufFdrTable, sizeof(int32_t) * (kFdrBuckets + 1), &ptab);
+    if (!rc) rc = ensure(d, kBufFdrInfo, sizeof(FdrInfo<T>), &pinfo);
     if (rc) return rc;
-    T *factor = (T *)pf, *maxv = (T *)pmax, *minv = (T *)pmin;
+    b->factor = (T *)pf; b->maxv = (T *)pmax; b->minv = (T *)pmin; b->thr = (T *)pthr;
+    b->hist = (unsigned long long *)phist; b->table = (int32_t *)ptab;
+    b->info = (FdrInfo<T> *)pinfo;
+    typedef typename KeyOf<T>::type K;
+    int np = 2;
+    while (np < G) np <<= 1;
+    b->npow2 = (size_t)np * sizeof(K) <= (size_t)d->lds_max && np <= 32768 ? np : 0;
+    if (b->npow2) {
+        HIP_TRY(hipFuncSetAttribute((const void *)k_sort_thresholds<T>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(b->npow2 * sizeof(K))));
+    }
+    const size_t lds = align16(sizeof(T) * G) + align16(sizeof(int32_t) * (kFdrBuckets + 1)) +
+                       2 * sizeof(uint32_t) * (G + 1);
+    b->fdr_lds = lds <= (size_t)d->lds_max ? lds : 0;
+    if (b->fdr_lds) {
+        HIP_TRY(hipFuncSetAttribute((const void *)k_fdr_hist<T, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    return GSEA_OK;
+}
+
+// the sorted thresholds and bucket table computed on the host (more sets
+// than the device sort holds): the same values k_sort_thresholds writes
+template <class T>
+static int host_thresholds(const T *d_stats, int64_t G, const StatsBufs<T> &b,
+                           hipStream_t st) {
+    std::vector<T> nes((size_t)G);
+    HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<T> thr;
+    thr.reserve((size_t)G);
+    for (T v : nes)
+        if (v == v) thr.push_back(v);
+    std::sort(thr.begin(), thr.end());
+    FdrInfo<T> f{};
+    f.n = (int)thr.size();
+    std::vector<int32_t> table;
+    if (f.n > 1 && thr.back() > thr.front()) {
+        f.nb = std::min(kFdrBuckets, f.n);
+        const double w = ((double)thr.back() - (double)thr.front()) / f.nb;
+        f.lo = thr.front();
+        f.inv_w = (T)(1.0 / w);
+        table.resize((size_t)f.nb + 1);
+        for (int bk = 0; bk <= f.nb; ++bk) {
+            const double edge = (double)thr.front() + bk * w;
+            table[(size_t)bk] = (int32_t)(std::lower_bound(thr.begin(), thr.end(), edge,
+                                                           [](T x, double e) {
+                                                               return (double)x < e;
+                                                           }) - thr.begin());
+        }
+        table[0] = 0;
+        table[(size_t)f.nb] = f.n;
+    } else if (f.n) {
+        f.lo = thr.front();
+    }
+    if (f.n)
+        HIP_TRY(hipMemcpyAsync(b.thr, thr.data(), sizeof(T) * f.n, hipMemcpyHostToDevice, st));
+    if (f.nb)
+        HIP_TRY(hipMemcpyAsync(b.table, table.data(), sizeof(int32_t) * (f.nb + 1),
+                               hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.info, &f, sizeof(f), hipMemcpyHostToDevice, st));
+    // the host vectors above feed asynchronous copies
+    HIP_TRY(hipStreamSynchronize(st));
+    return GSEA_OK;
+}
+
+// ES, NES, NP, FWER and FDR of es[P][G] into d_stats [5][G]
+// (statistics.cuh:46-117; FDR defined by this project), queued on `st`.
+// With the device threshold sort (b.npow2 > 0) nothing here waits for the
+// device or reads device memory on the host, so the sequence can be captured
+// into a graph.  budget (optional): set s uses permutations 0..budget[s]-1.
+template <class T>
+static int enqueue_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
+                         const StatsBufs<T> &b, const DeviceState *d, hipStream_t st,
+                         const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
     const int g = (int)G;
     // one wavefront per 64 sets: the sequential per-set sums are latency
     // bound, so spread them over as many CUs as possible
     hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 63) / 64), dim3(64), 0, st,
-                       es_pg, P, g, d_stats, factor, d_budget, d_cv);
+                       es_pg, P, g, d_stats, b.factor, d_budget, d_cv);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_perm_extrema<T>, dim3((unsigned)((P + 3) / 4)), dim3(256), 0,
-                       st, es_pg, P, g, factor, d_budget, maxv, minv);
+                       st, es_pg, P, g, b.factor, d_budget, b.maxv, b.minv);
     HIP_TRY(hipGetLastError());
-    // host buffers of this function feed asynchronous copies: the function
-    // synchronises the stream before they go out of scope
-    std::vector<T> nes(G), ext;
-    std::vector<int32_t> table;
-    if (d_budget) {
-        hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
-                           P, g, maxv, minv, d_budget);
+    // sorted observed NES (FDR thresholds, FWER through histograms)
+    if (b.npow2) {
+        typedef typename KeyOf<T>::type K;
+        hipLaunchKernelGGL(k_sort_thresholds<T>, dim3(1), dim3(1024), b.npow2 * sizeof(K), st,
+                           (const T *)d_stats, g, b.npow2, b.thr, b.info, b.table);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
-                               hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
     } else {
-        // FWER by binary search in the sorted per-permutation extrema
-        ext.resize((size_t)2 * P);
-        HIP_TRY(hipMemcpyAsync(ext.data(), maxv, sizeof(T) * P, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(ext.data() + P, minv, sizeof(T) * P, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(nes.data(), d_stats + G, sizeof(T) * G,
-                               hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        std::sort(ext.begin(), ext.begin() + P);
-        std::sort(ext.begin() + P, ext.end());
-        void *psorted;
-        rc = ensure(d, kBufExtSorted, sizeof(T) * 2 * P, &psorted);
+        int rc = host_thresholds<T>(d_stats, G, b, st);
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(psorted, ext.data(), sizeof(T) * 2 * P,
-                               hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_fwer_sorted<T>, dim3((g + 255) / 256), dim3(256), 0, st,
-                           d_stats, P, g, (const T *)psorted, (const T *)psorted + P);
+    }
+    unsigned long long *hub = b.hist, *hlb = hub + (G + 1), *signs = hlb + (G + 1);
+    unsigned long long *hmax = signs + 2, *hmin = hmax + (G + 1);
+    const int64_t hl = hist_len(G);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::min<int64_t>((hl + 255) / 256, 1024)),
+                       dim3(256), 0, st, b.hist, hl);
+    HIP_TRY(hipGetLastError());
+    if (d_budget) {
+        // each set over its own permutations
+        hipLaunchKernelGGL(k_fwer<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
+                           P, g, b.maxv, b.minv, d_budget);
+        HIP_TRY(hipGetLastError());
+    } else {
+        const unsigned blocks = (unsigned)std::min<int64_t>((P + 255) / 256, 4LL * d->cu_count);
+        hipLaunchKernelGGL(k_fwer_hist<T>, dim3(blocks), dim3(256), 0, st, (const T *)b.maxv,
+                           (const T *)b.minv, P, (const T *)b.thr,
+                           (const FdrInfo<T> *)b.info, hmax, hmin);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_prefix_u64, dim3(1), dim3(64), 0, st, hmax, hmin,
+                           (const int *)&b.info->n);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_fwer_final<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
+                           P, g, (const T *)b.thr, (const FdrInfo<T> *)b.info,
+                           (const unsigned long long *)hmax, (const unsigned long long *)hmin);
         HIP_TRY(hipGetLastError());
     }
-
-    // FDR: thresholds are the sorted non-NaN NES (sorted on the host: G values)
-    std::vector<T> thr;
-    thr.reserve(G);
-    for (T v : nes)
-        if (v == v) thr.push_back(v);
-    std::sort(thr.begin(), thr.end());
-    const int n = (int)thr.size();
-    void *pthr, *phist;
-    rc = ensure(d, kBufThr, sizeof(T) * std::max(n, 1), &pthr);
-    if (!rc) rc = ensure(d, kBufHist, sizeof(unsigned long long) * (2 * (n + 1) + 2), &phist);
-    if (rc) return rc;
-    unsigned long long *hub = (unsigned long long *)phist;
-    unsigned long long *hlb = hub + (n + 1);
-    unsigned long long *signs = hlb + (n + 1);
-    if (n) HIP_TRY(hipMemcpyAsync(pthr, thr.data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(phist, 0, sizeof(unsigned long long) * (2 * (n + 1) + 2), st));
     if (P > 1) {
-        // bucket table for the threshold search: table[b] = #thresholds below
-        // lo + b*w, nb uniform buckets over [thr[0], thr[n-1]]
-        int nb = 0;
-        T lo = T(0), inv_w = T(0);
-        if (n > 1 && thr[n - 1] > thr[0]) {
-            nb = (int)std::min<int64_t>(4096, n);
-            lo = thr[0];
-            const double w = ((double)thr[n - 1] - (double)thr[0]) / nb;
-            inv_w = (T)(1.0 / w);
-            table.resize((size_t)nb + 1);
-            for (int b = 0; b <= nb; ++b) {
-                const double edge = (double)thr[0] + b * w;
-                table[(size_t)b] = (int32_t)(std::lower_bound(thr.begin(), thr.end(), edge,
-                                                              [](T x, double e) {
-                                                                  return (double)x < e;
-                                                              }) - thr.begin());
-            }
-            table[0] = 0;                  // bounds, whatever the edges' rounding
-            table[(size_t)nb] = n;
-        }
-        void *ptab;
-        rc = ensure(d, kBufFdrTable, sizeof(int32_t) * (size_t)(nb + 1), &ptab);
-        if (rc) return rc;
-        if (nb)
-            HIP_TRY(hipMemcpyAsync(ptab, table.data(), sizeof(int32_t) * (nb + 1),
-                                   hipMemcpyHostToDevice, st));
-        const size_t lds = align16(sizeof(T) * n) + align16(sizeof(int32_t) * (nb + 1)) +
-                           2 * sizeof(uint32_t) * (n + 1);
         const int blocks = (int)std::min<int64_t>(P - 1, 4LL * d->cu_count);
-        if (lds <= (size_t)d->lds_max) {
-            HIP_TRY(hipFuncSetAttribute((const void *)k_fdr_hist<T, true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-            hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), lds, st,
-                               es_pg, P, g, factor, d_budget, (const T *)pthr, n,
-                               (const int32_t *)ptab, nb, lo, inv_w, hub, hlb, signs);
+        if (b.fdr_lds) {
+            hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), b.fdr_lds, st,
+                               es_pg, P, g, (const T *)b.factor, d_budget, (const T *)b.thr,
+                               (const FdrInfo<T> *)b.info, (const int32_t *)b.table, hub, hlb,
+                               signs);
         } else {
             hipLaunchKernelGGL((k_fdr_hist<T, false>), dim3(blocks), dim3(256), 0, st,
-                               es_pg, P, g, factor, d_budget, (const T *)pthr, n,
-                               (const int32_t *)ptab, 0, lo, inv_w, hub, hlb, signs);
+                               es_pg, P, g, (const T *)b.factor, d_budget, (const T *)b.thr,
+                               (const FdrInfo<T> *)b.info, (const int32_t *)b.table, hub, hlb,
+                               signs);
         }
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_prefix_u64, dim3(1), dim3(64), 0, st, hub, hlb, n + 1);
+    hipLaunchKernelGGL(k_prefix_u64, dim3(1), dim3(64), 0, st, hub, hlb,
+                       (const int *)&b.info->n);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_fdr_final<T>, dim3((g + 255) / 256), dim3(256), 0, st, d_stats,
-                       g, (const T *)pthr, n, hub, hlb, signs);
+                       g, (const T *)b.thr, (const FdrInfo<T> *)b.info,
+                       (const unsigned long long *)hub, (const unsigned long long *)hlb,
+                       (const unsigned long long *)signs);
     HIP_TRY(hipGetLastError());
+    return GSEA_OK;
+}
+
+template <class T>
+static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
+                     DeviceState *d, hipStream_t st,
+                     const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
+    StatsBufs<T> b;
+    int rc = prepare_stats<T>(P, G, d, &b);
+    if (!rc) rc = enqueue_stats<T>(es_pg, P, G, d_stats, b, d, st, d_budget, d_cv);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     return GSEA_OK;
 }
@@ -721,6 +844,130 @@ static int64_t batch_perms(int64_t N, int64_t P, size_t w) {
     return std::min<int64_t>(P, cap);
 }
 
+// Everything a run needs before its first launch: buffers, the labels on
+// the device, the reciprocal table, the set tiers and the statistics'
+// buffers.  May wait for the device; enqueue_run then queues the whole path
+// without waiting and without reading device memory on the host.
+template <class T>
+struct RunCtx {
+    DeviceState *d = nullptr;
+    const T *dX = nullptr;
+    int64_t N = 0, S = 0, G = 0, P = 0, Pb = 0;
+    int flags = 0;
+    const int32_t *d_off = nullptr, *d_genes = nullptr;
+    Labels L;
+    uint8_t *plab = nullptr, *psig = nullptr;
+    uint16_t *plists = nullptr;
+    T *pr = nullptr, *pes = nullptr;
+    SetTiers tiers;
+    StatsBufs<T> sb;
+};
+
+template <class T>
+static int prepare_run(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                       const int32_t *d_off, const int32_t *d_genes, int64_t G, int64_t P,
+                       int flags, hipStream_t st, RunCtx<T> *c) {
+    if (!dX || !labels || !d_off || !d_genes)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    int rc = device_state(&c->d);
+    if (rc) return rc;
+    DeviceState *d = c->d;
+    rc = prepare_labels(labels, S, flags, &c->L);
+    if (rc) return rc;
+    c->dX = dX; c->N = N; c->S = S; c->G = G; c->P = P; c->flags = flags;
+    c->d_off = d_off; c->d_genes = d_genes;
+    c->Pb = batch_perms(N, P, sizeof(T));
+    void *plab, *psig, *plists, *pr, *pes;
+    rc = ensure(d, kBufLabels, (size_t)S, &plab);
+    if (!rc) rc = ensure(d, kBufSig, (size_t)c->Pb * S, &psig);
+    if (!rc) rc = ensure(d, kBufLists, (size_t)c->Pb * c->L.lstride * 2, &plists);
+    if (!rc) rc = ensure(d, kBufR, (size_t)c->Pb * N * sizeof(T), &pr);
+    if (!rc) rc = ensure(d, kBufES, (size_t)P * G * sizeof(T), &pes);
+    if (rc) return rc;
+    c->plab = (uint8_t *)plab; c->psig = (uint8_t *)psig; c->plists = (uint16_t *)plists;
+    c->pr = (T *)pr; c->pes = (T *)pes;
+    HIP_TRY(hipMemcpyAsync(plab, c->L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
+    // 1/(i+1) for the Welford running counts, correctly rounded doubles
+    if (d->rcp_len < S) {
+        std::vector<double> rcp((size_t)S);
+        for (int64_t i = 0; i < S; ++i) rcp[(size_t)i] = 1.0 / (double)(i + 1);
+        void *prc;
+        rc = ensure(d, kBufRcp, sizeof(double) * S, &prc);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(prc, rcp.data(), sizeof(double) * S, hipMemcpyHostToDevice));
+        d->rcp_len = S;
+    }
+    rc = cached_tiers<T>(d_off, d_genes, G, N, d, st, &c->tiers);
+    if (!rc) rc = prepare_stats<T>(P, G, d, &c->sb);
+    if (rc) return rc;
+    // rank rows handed from k_rank to k_es
+    void *pa, *pp;
+    rc = ensure(d, kBufAGlob, (size_t)c->Pb * N * sizeof(T), &pa);
+    if (!rc) rc = ensure(d, kBufPos, (size_t)c->Pb * N * sizeof(uint16_t), &pp);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));   // the labels' host copy
+    return GSEA_OK;
+}
+
+// class lists of permutations [base, base + pb)
+template <class T>
+static int enqueue_lists(const RunCtx<T> &c, int64_t base, int64_t pb, hipStream_t st) {
+    hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((pb + 63) / 64)), dim3(64), 0, st,
+                       (const uint8_t *)c.plab, (int)c.S, base, (int)pb, c.psig, c.L.offB,
+                       c.plists, c.L.lstride);
+    HIP_TRY(hipGetLastError());
+    return GSEA_OK;
+}
+
+// The whole path for one metric: per permutation batch class lists, metric,
+// rank rows and enrichment scores (es[P][G]), then the statistics and, when
+// asked, the [G][P] matrix.  lists_ready: the class lists of a single batch
+// are already in place (a sweep over metrics computes them once).  clk
+// (optional) brackets the stages with events.
+template <class T>
+static int enqueue_run(const RunCtx<T> &c, int metric, T *d_stats, T *d_es_out,
+                       hipStream_t st, StageClock *clk, bool lists_ready, int64_t *batches) {
+    int rc;
+    int64_t nb = 0;
+    const double *rcp = (const double *)c.d->bufs[kBufRcp].ptr;
+    for (int64_t base = 0; base < c.P; base += c.Pb, ++nb) {
+        const int64_t pb = std::min(c.Pb, c.P - base);
+        if (!(lists_ready && c.Pb >= c.P)) {
+            if (clk && (rc = clk->begin(0))) return rc;
+            if ((rc = enqueue_lists<T>(c, base, pb, st))) return rc;
+            if (clk && (rc = clk->end())) return rc;
+        }
+        if (clk && (rc = clk->begin(1))) return rc;
+        rc = launch_metric_rcp<T>(c.dX, c.N, c.S, c.L, c.plists, pb, metric, c.pr, rcp, c.d, st);
+        if (rc) return rc;
+        if (clk && (rc = clk->end())) return rc;
+        if (clk && (rc = clk->begin(2))) return rc;
+        rc = launch_rank<T>((const T *)c.pr, c.N, pb, base, c.d_off, c.d_genes, c.G, c.flags,
+                            c.pes, c.d, st, nullptr);
+        if (rc) return rc;
+        if (clk && (rc = clk->end())) return rc;
+        if (clk && (rc = clk->begin(5))) return rc;
+        rc = launch_es_rows<T>(c.N, pb, base, c.d_off, c.d_genes, c.G, c.flags, c.pes, c.d, st,
+                               &c.tiers, 0, pb);
+        if (rc) return rc;
+        if (clk && (rc = clk->end())) return rc;
+    }
+    if (batches) *batches = nb;
+    if (clk && (rc = clk->begin(3))) return rc;
+    rc = enqueue_stats<T>((const T *)c.pes, c.P, c.G, d_stats, c.sb, c.d, st);
+    if (rc) return rc;
+    if (clk && (rc = clk->end())) return rc;
+    if (d_es_out) {
+        if (clk && (rc = clk->begin(4))) return rc;
+        hipLaunchKernelGGL(k_transpose<T>,
+                           dim3((unsigned)((c.G + 31) / 32), (unsigned)((c.P + 31) / 32)),
+                           dim3(32, 8), 0, st, (const T *)c.pes, c.P, (int)c.G, d_es_out);
+        HIP_TRY(hipGetLastError());
+        if (clk && (rc = clk->end())) return rc;
+    }
+    return GSEA_OK;
+}
+
 template <class T>
 static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
                       const int32_t *d_off, const int32_t *d_genes, int64_t G,
@@ -728,67 +975,156 @@ static int run_device(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
                       hipStream_t st, gsea_timings *t) {
     int rc = check_sizes(N, S, G, P, metric);
     if (rc) return rc;
-    if (!dX || !labels || !d_off || !d_genes || !d_stats)
-        return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
-    DeviceState *d = nullptr;
-    rc = device_state(&d);
+    if (!d_stats) return fail(GSEA_ERR_INVALID_ARGUMENT, "null pointer argument");
+    RunCtx<T> c;
+    rc = prepare_run<T>(dX, N, S, labels, d_off, d_genes, G, P, flags, st, &c);
     if (rc) return rc;
-    Labels L;
-    rc = prepare_labels(labels, S, flags, &L);
-    if (rc) return rc;
-
-    const int64_t Pb = batch_perms(N, P, sizeof(T));
-    void *plab, *psig, *plists, *pr, *pes;
-    rc = ensure(d, kBufLabels, (size_t)S, &plab);
-    if (!rc) rc = ensure(d, kBufSig, (size_t)Pb * S, &psig);
-    if (!rc) rc = ensure(d, kBufLists, (size_t)Pb * L.lstride * 2, &plists);
-    if (!rc) rc = ensure(d, kBufR, (size_t)Pb * N * sizeof(T), &pr);
-    if (!rc) rc = ensure(d, kBufES, (size_t)P * G * sizeof(T), &pes);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(plab, L.lab.data(), (size_t)S, hipMemcpyHostToDevice, st));
-
-    SetTiers tiers;
-    rc = build_tiers<T>(d_off, d_genes, G, N, d, st, &tiers);
-    if (rc) return rc;
-    StageClock clk{d, st};
+    StageClock clk{c.d, st};
     int64_t batches = 0;
-    for (int64_t base = 0; base < P; base += Pb, ++batches) {
-        const int64_t pb = std::min(Pb, P - base);
-        if ((rc = clk.begin(0))) return rc;
-        hipLaunchKernelGGL(k_class_lists, dim3((unsigned)((pb + 63) / 64)), dim3(64), 0, st,
-                           (const uint8_t *)plab, (int)S, base, (int)pb, (uint8_t *)psig,
-                           L.offB, (uint16_t *)plists, L.lstride);
-        HIP_TRY(hipGetLastError());
-        if ((rc = clk.end())) return rc;
-        if ((rc = clk.begin(1))) return rc;
-        rc = launch_metric<T>(dX, N, S, L, (const uint16_t *)plists, pb, metric,
-                              (T *)pr, d, st);
-        if (rc) return rc;
-        if ((rc = clk.end())) return rc;
-        if ((rc = clk.begin(2))) return rc;
-        rc = launch_rank<T>((const T *)pr, N, pb, base, d_off, d_genes, G, flags,
-                            (T *)pes, d, st, nullptr);
-        if (rc) return rc;
-        if ((rc = clk.end())) return rc;
-        if ((rc = clk.begin(5))) return rc;
-        rc = launch_es_rows<T>(N, pb, base, d_off, d_genes, G, flags, (T *)pes, d, st,
-                               &tiers, 0, pb);
-        if (rc) return rc;
-        if ((rc = clk.end())) return rc;
-    }
-    if ((rc = clk.begin(3))) return rc;
-    rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st);
+    rc = enqueue_run<T>(c, metric, d_stats, d_es_out, st, &clk, false, &batches);
     if (rc) return rc;
-    if ((rc = clk.end())) return rc;
-    if (d_es_out) {
-        if ((rc = clk.begin(4))) return rc;
-        hipLaunchKernelGGL(k_transpose<T>, dim3((unsigned)((G + 31) / 32), (unsigned)((P + 31) / 32)),
-                           dim3(32, 8), 0, st, (const T *)pes, P, (int)G, d_es_out);
-        HIP_TRY(hipGetLastError());
-        if ((rc = clk.end())) return rc;
-    }
     HIP_TRY(hipStreamSynchronize(st));
     return finish_timings(clk, t, batches);
+}
+
+// ---------------------------------------------------------------------------
+// sweep over metrics: one HIP graph per shape
+// ---------------------------------------------------------------------------
+struct SweepKey {
+    const void *dX = nullptr;
+    int64_t N = 0, S = 0, G = 0, P = 0;
+    std::vector<uint8_t> labels;
+    const int32_t *off = nullptr, *genes = nullptr;
+    std::vector<int> metrics;
+    int flags = 0;
+    const void *stats = nullptr;
+    hipStream_t st = nullptr;
+    size_t tsize = 0;
+    gsea_tuning tune = {};
+    TierKey tiers;
+    uint64_t generation = 0;
+
+    bool operator==(const SweepKey &o) const {
+        const TierKey &a = tiers, &b = o.tiers;
+        return dX == o.dX && N == o.N && S == o.S && G == o.G && P == o.P &&
+               labels == o.labels && off == o.off && genes == o.genes &&
+               metrics == o.metrics && flags == o.flags && stats == o.stats && st == o.st &&
+               tsize == o.tsize && std::memcmp(&tune, &o.tune, sizeof(tune)) == 0 &&
+               a.sum == b.sum && a.total == b.total && a.lane_max == b.lane_max &&
+               generation == o.generation;
+    }
+};
+
+struct SweepGraph {
+    bool valid = false;
+    SweepKey key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static SweepGraph g_sweep[64];
+
+static void drop_graph(SweepGraph &sg) {
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    if (sg.graph) (void)hipGraphDestroy(sg.graph);
+    sg.exec = nullptr;
+    sg.graph = nullptr;
+    sg.valid = false;
+}
+
+// The whole path for several metrics over the same inputs: class lists once
+// (a single permutation batch), then per metric the metric rows, rank rows,
+// enrichment scores and statistics into d_stats[m][5][G].  With use_graph
+// the sequence is captured into one HIP graph on the first call and replayed
+// on later calls with the same key; otherwise (or when the statistics need
+// the host: more sets than the device sort holds) it is queued directly.
+template <class T>
+static int run_sweep(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
+                     const int32_t *d_off, const int32_t *d_genes, int64_t G, int64_t P,
+                     const int *metrics, int nm, int flags, T *d_stats, hipStream_t st,
+                     bool use_graph, gsea_timings *t) {
+    if (!metrics || nm < 1 || !d_stats)
+        return fail(GSEA_ERR_INVALID_ARGUMENT, "no metrics or null pointer argument");
+    for (int m = 0; m < nm; ++m) {
+        int rc = check_sizes(N, S, G, P, metrics[m]);
+        if (rc) return rc;
+    }
+    DeviceState *d = nullptr;
+    int rc = device_state(&d);
+    if (rc) return rc;
+    int dev = 0;
+    if ((rc = current_device(&dev))) return rc;
+    if (!st) {
+        // a blocking stream: ordered after the null stream's earlier work and
+        // before its later work, and capturable
+        if (!d->own_stream) HIP_TRY(hipStreamCreate(&d->own_stream));
+        st = d->own_stream;
+    }
+    RunCtx<T> c;
+    rc = prepare_run<T>(dX, N, S, labels, d_off, d_genes, G, P, flags, st, &c);
+    if (rc) return rc;
+    const bool one_batch = c.Pb >= P;
+    const bool capturable = use_graph && c.sb.npow2 > 0;
+    auto enqueue_all = [&]() -> int {
+        int r;
+        if (one_batch && (r = enqueue_lists<T>(c, 0, P, st))) return r;
+        for (int m = 0; m < nm; ++m) {
+            r = enqueue_run<T>(c, metrics[m], d_stats + (size_t)m * 5 * G, nullptr, st, nullptr,
+                               one_batch, nullptr);
+            if (r) return r;
+        }
+        return GSEA_OK;
+    };
+    hipEvent_t e0, e1;
+    StageClock clk{d, st};
+    int i0, i1;
+    if ((rc = clk.event(&i0))) return rc;
+    e0 = d->events[i0];
+    if (!capturable) {
+        if ((rc = enqueue_all())) return rc;
+    } else {
+        SweepKey k;
+        k.dX = dX; k.N = N; k.S = S; k.G = G; k.P = P;
+        k.labels.assign(labels, labels + S);
+        k.off = d_off; k.genes = d_genes;
+        k.metrics.assign(metrics, metrics + nm);
+        k.flags = flags; k.stats = d_stats; k.st = st; k.tsize = sizeof(T);
+        k.tune = g_tune;
+        k.tiers = d->tier_key;
+        k.generation = g_ws_generation;
+        SweepGraph &sg = g_sweep[dev];
+        if (!(sg.valid && sg.key == k)) {
+            drop_graph(sg);
+            // the event above is recorded outside the capture
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            rc = enqueue_all();
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess)
+                return fail(GSEA_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                              hipGetErrorString(ec));
+            sg.graph = g;
+            HIP_TRY(hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0));
+            sg.key = k;
+            sg.valid = g_ws_generation == k.generation;
+        }
+        HIP_TRY(hipGraphLaunch(sg.exec, st));
+    }
+    if ((rc = clk.event(&i1))) return rc;
+    e1 = d->events[i1];
+    HIP_TRY(hipStreamSynchronize(st));
+    if (t) {
+        std::memset(t, 0, sizeof(*t));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        t->ms_total = ms;
+        t->perm_batches = (P + c.Pb - 1) / c.Pb;
+    }
+    return GSEA_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1342,6 +1678,24 @@ int gsea_run_device_f64(const double *dX, int64_t N, int64_t S, const uint8_t *l
     std::lock_guard<std::mutex> lk(g_mu);
     return run_device<double>(dX, N, S, labels, d_off, d_genes, G, P, metric, flags,
                               d_es_out, d_stats, (hipStream_t)stream, t);
+}
+
+int gsea_sweep_device_f32(const float *dX, int64_t N, int64_t S, const uint8_t *labels,
+                          const int32_t *d_off, const int32_t *d_genes, int64_t G, int64_t P,
+                          const int *metric_ids, int num_metrics, int flags, int use_graph,
+                          float *d_stats, void *stream, gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_sweep<float>(dX, N, S, labels, d_off, d_genes, G, P, metric_ids, num_metrics,
+                            flags, d_stats, (hipStream_t)stream, use_graph != 0, t);
+}
+
+int gsea_sweep_device_f64(const double *dX, int64_t N, int64_t S, const uint8_t *labels,
+                          const int32_t *d_off, const int32_t *d_genes, int64_t G, int64_t P,
+                          const int *metric_ids, int num_metrics, int flags, int use_graph,
+                          double *d_stats, void *stream, gsea_timings *t) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return run_sweep<double>(dX, N, S, labels, d_off, d_genes, G, P, metric_ids, num_metrics,
+                             flags, d_stats, (hipStream_t)stream, use_graph != 0, t);
 }
 
 int gsea_run_budget_device_f32(const float *dX, int64_t N, int64_t S,

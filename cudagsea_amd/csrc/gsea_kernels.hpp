@@ -93,6 +93,40 @@ __global__ void k_small_gm(const int32_t *__restrict__ off,
 // statistics.cuh:60-70), NP (statistics.cuh:73-77).  budget (optional):
 // set s was scored for permutations 0..budget[s]-1 only and its statistics
 // use those; cv (optional, with budget) receives std/mean of |null ES|.
+// order-sensitive 64-bit checksum of a set CSR (offsets[G+1], then the
+// offsets[G] gene ids), one workgroup: identifies a set list whose derived
+// work tables can be reused
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(1024) void k_csr_checksum(const int32_t *__restrict__ off,
+                                                       const int32_t *__restrict__ genes,
+                                                       int64_t G,
+                                                       unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long part[1024 / kWave];
+    const int64_t total = off[G];
+    unsigned long long h = 0;
+    for (int64_t i = threadIdx.x; i <= G; i += blockDim.x)
+        h += mix64(((unsigned long long)(uint32_t)off[i] << 32) ^ (unsigned long long)i);
+    for (int64_t i = threadIdx.x; i < total; i += blockDim.x)
+        h += mix64(((unsigned long long)(uint32_t)genes[i] << 32) ^
+                   (0x8000000000000000ull | (unsigned long long)i));
+    h = wave_sum(h);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += part[w];
+        out[0] = t;
+        out[1] = (unsigned long long)total;
+    }
+}
+
 template <class T>
 __global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
                             T *__restrict__ out, T *__restrict__ factor,
@@ -184,42 +218,6 @@ __global__ void k_fwer(T *__restrict__ out, int64_t P, int G,
     out[3 * G + s] = (T)c / (T)B;
 }
 
-// FWER from the per-permutation extrema sorted ascending (same counts as
-// k_fwer's loop: #max >= NES for NES > 0, #min <= NES otherwise; NaN NES
-// counts nothing)
-template <class T>
-__device__ __forceinline__ int64_t count_ge(const T *v, int64_t n, T x) {
-    int64_t lo = 0, hi = n;   // first index with v >= x
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (v[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return n - lo;
-}
-
-template <class T>
-__device__ __forceinline__ int64_t count_le(const T *v, int64_t n, T x) {
-    int64_t lo = 0, hi = n;   // first index with v > x
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (v[mid] <= x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-template <class T>
-__global__ void k_fwer_sorted(T *__restrict__ out, int64_t P, int G,
-                              const T *__restrict__ max_sorted,
-                              const T *__restrict__ min_sorted) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= G) return;
-    const T score = out[G + s];
-    int64_t c = 0;
-    if (score > T(0)) c = count_ge(max_sorted, P, score);
-    else if (score == score) c = count_le(min_sorted, P, score);
-    out[3 * G + s] = (T)c / (T)P;
-}
-
 template <class T>
 __device__ __forceinline__ int lower_bound_t(const T *v, int n, T x) {
     int lo = 0, hi = n;
@@ -240,6 +238,158 @@ __device__ __forceinline__ int upper_bound_t(const T *v, int n, T x) {
     return lo;
 }
 
+// Sorted observed NES and the bucket table that bounds searches in them,
+// produced on the device by k_sort_thresholds (or uploaded by the host when
+// there are more sets than one workgroup sorts)
+template <class T>
+struct FdrInfo {
+    int n;      // non-NaN NES, sorted ascending in thr[0..n)
+    int nb;     // buckets of the table (0: no table)
+    T lo;       // thr[0]
+    T inv_w;    // buckets per unit of NES
+};
+
+constexpr int kFdrBuckets = 4096;
+
+// zero fill (stream-ordered, graph-safe: no memset nodes)
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+template <class K>
+__device__ __forceinline__ K key_max();
+template <>
+__device__ __forceinline__ uint32_t key_max<uint32_t>() { return 0xffffffffu; }
+template <>
+__device__ __forceinline__ uint64_t key_max<uint64_t>() { return ~0ull; }
+
+// value of an ascending sort key (sort_key(v, true)); NaN keys are not
+// converted back
+__device__ __forceinline__ float key_value(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ double key_value(uint64_t u) {
+    return __longlong_as_double((long long)((u & 0x8000000000000000ull)
+                                                ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+// One workgroup: the NES row of stats [5][G] sorted ascending (bitonic
+// network over npow2 >= G keys in LDS, NaN and padding last), the non-NaN
+// values written to thr[0..n), and the bucket table of the FDR search:
+// nb = min(4096, n) uniform buckets over [thr[0], thr[n-1]] (width w in
+// double), table[b] = #thresholds below thr[0] + b w, table[0] = 0,
+// table[nb] = n; no table when n < 2 or all thresholds are equal.
+template <class T>
+__global__ __launch_bounds__(1024) void k_sort_thresholds(const T *__restrict__ stats, int G,
+                                                          int npow2, T *__restrict__ thr,
+                                                          FdrInfo<T> *__restrict__ info,
+                                                          int32_t *__restrict__ table) {
+    typedef typename KeyOf<T>::type K;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    K *key = reinterpret_cast<K *>(smem_raw);
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int i = threadIdx.x; i < npow2; i += blockDim.x) {
+        K k = key_max<K>();
+        if (i < G) {
+            const T v = stats[G + i];
+            k = (K)sort_key(v, true);   // NaN canonicalised above +inf
+            mine += v == v;
+        }
+        key[i] = k;
+    }
+    atomicAdd(&cnt, mine);
+    for (int size = 2; size <= npow2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < (npow2 >> 1); t += blockDim.x) {
+                const int i = 2 * stride * (t / stride) + (t % stride);
+                const int j = i + stride;
+                const K x = key[i], y = key[j];
+                const bool up = (i & size) == 0;
+                if ((x > y) == up) { key[i] = y; key[j] = x; }
+            }
+        }
+    }
+    __syncthreads();
+    const int n = cnt;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) thr[i] = (T)key_value(key[i]);
+    int nb = 0;
+    double w = 0.0;
+    if (n > 1) {
+        const T t0 = (T)key_value(key[0]), t1 = (T)key_value(key[n - 1]);
+        if (t1 > t0) {
+            nb = n < kFdrBuckets ? n : kFdrBuckets;
+            w = ((double)t1 - (double)t0) / nb;
+        }
+    }
+    if (nb) {
+        const double t0 = (double)(T)key_value(key[0]);
+        for (int b = threadIdx.x; b <= nb; b += blockDim.x) {
+            int v;
+            if (b == 0) {
+                v = 0;
+            } else if (b == nb) {
+                v = n;
+            } else {
+                const double edge = t0 + b * w;
+                int lo = 0, hi = n;   // first threshold with (double)x >= edge
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((double)(T)key_value(key[mid]) < edge) lo = mid + 1; else hi = mid;
+                }
+                v = lo;
+            }
+            table[b] = v;
+        }
+    }
+    if (threadIdx.x == 0) {
+        FdrInfo<T> f;
+        f.n = n;
+        f.nb = nb;
+        f.lo = n ? (T)key_value(key[0]) : T(0);
+        f.inv_w = nb ? (T)(1.0 / w) : T(0);
+        *info = f;
+    }
+}
+
+// FWER counts through the sorted NES: for every permutation p (0 included),
+// hmax[#thresholds <= max_p] and hmin[#thresholds < min_p] are counted.  A set
+// with NES > 0 then has #{p : max_p >= NES} = P - (prefix of hmax at
+// lower_bound(NES)), one with NES <= 0 #{p : min_p <= NES} = prefix of hmin
+// at lower_bound(NES) -- the counts of statistics.cuh:100-110.
+template <class T>
+__global__ void k_fwer_hist(const T *__restrict__ maxv, const T *__restrict__ minv, int64_t P,
+                            const T *__restrict__ thr, const FdrInfo<T> *__restrict__ info,
+                            unsigned long long *__restrict__ hmax,
+                            unsigned long long *__restrict__ hmin) {
+    const int n = info->n;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        atomicAdd(&hmax[upper_bound_t(thr, n, maxv[p])], 1ull);
+        atomicAdd(&hmin[lower_bound_t(thr, n, minv[p])], 1ull);
+    }
+}
+
+template <class T>
+__global__ void k_fwer_final(T *__restrict__ out, int64_t P, int G, const T *__restrict__ thr,
+                             const FdrInfo<T> *__restrict__ info,
+                             const unsigned long long *__restrict__ cmax,
+                             const unsigned long long *__restrict__ cmin) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G) return;
+    const int n = info->n;
+    const T score = out[G + s];
+    int64_t c = 0;
+    if (score > T(0)) c = P - (int64_t)cmax[lower_bound_t(thr, n, score)];
+    else if (score == score) c = (int64_t)cmin[lower_bound_t(thr, n, score)];
+    out[3 * G + s] = (T)c / (T)P;
+}
+
 // FDR histograms.  thr[0..n) are the sorted non-NaN NES.  For every null
 // score z (permutations 1..P-1, normalised as statistics.cuh:90): hub[#thr<=z]
 // and hlb[#thr<z] are counted, plus #z>=0 and #z<=0.
@@ -249,20 +399,23 @@ template <class T, bool LDS>
 __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
                            const T *__restrict__ factor,
                            const int32_t *__restrict__ budget,
-                           const T *__restrict__ thr, int n,
-                           const int32_t *__restrict__ table, int nb, T lo, T inv_w,
+                           const T *__restrict__ thr, const FdrInfo<T> *__restrict__ info,
+                           const int32_t *__restrict__ table,
                            unsigned long long *__restrict__ hub,
                            unsigned long long *__restrict__ hlb,
                            unsigned long long *__restrict__ signs) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = info->n, nb = LDS ? info->nb : 0;
+    const T lo = info->lo, inv_w = info->inv_w;
     const T *t = thr;
     uint32_t *h1 = nullptr, *h2 = nullptr;
     int32_t *tb = nullptr;
     if (LDS) {
+        // LDS layout sized by the host for G thresholds and the full table
         T *tl = reinterpret_cast<T *>(smem_raw);
-        tb = reinterpret_cast<int32_t *>(smem_raw + ((sizeof(T) * n + 15) / 16) * 16);
+        tb = reinterpret_cast<int32_t *>(smem_raw + ((sizeof(T) * G + 15) / 16) * 16);
         h1 = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(tb) +
-                                          ((sizeof(int32_t) * (nb + 1) + 15) / 16) * 16);
+                                          ((sizeof(int32_t) * (kFdrBuckets + 1) + 15) / 16) * 16);
         h2 = h1 + (n + 1);
         for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = thr[i];
         for (int i = threadIdx.x; i <= nb; i += blockDim.x) tb[i] = table[i];
@@ -334,9 +487,11 @@ __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
     }
 }
 
-// inclusive prefix sums of both histograms (single workgroup, in place)
+// inclusive prefix sums of both histograms (single workgroup, in place) over
+// *nptr + 1 entries
 __global__ void k_prefix_u64(unsigned long long *__restrict__ h1,
-                             unsigned long long *__restrict__ h2, int len) {
+                             unsigned long long *__restrict__ h2, const int *__restrict__ nptr) {
+    const int len = *nptr + 1;
     __shared__ unsigned long long carry[2];
     if (threadIdx.x == 0) { carry[0] = 0; carry[1] = 0; }
     __syncthreads();
@@ -366,12 +521,13 @@ __global__ void k_prefix_u64(unsigned long long *__restrict__ h1,
 // #z >= t = total - cum_hub[lb(t)], #z <= t = cum_hlb[lb(t)]
 template <class T>
 __global__ void k_fdr_final(T *__restrict__ out, int G,
-                            const T *__restrict__ thr, int n,
+                            const T *__restrict__ thr, const FdrInfo<T> *__restrict__ info,
                             const unsigned long long *__restrict__ cub,
                             const unsigned long long *__restrict__ clb,
                             const unsigned long long *__restrict__ signs) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= G) return;
+    const int n = info->n;
     const T nes = out[G + s];
     if (nes != nes) { out[4 * G + s] = (T)__builtin_nan(""); return; }
     const unsigned long long total = cub[n];

@@ -141,6 +141,28 @@ def run_device(X, labels, set_offsets, set_genes, num_perms, metric,
     return t.as_dict()
 
 
+def sweep_device(X, labels, set_offsets, set_genes, num_perms, metrics, stats_out,
+                 ascending=False, swap_labels=False, graph=True, stream=None):
+    """Several metrics over the same device inputs (gsea_sweep_device_*):
+    stats_out is a [len(metrics)][5][G] device tensor.  With graph=True the
+    launch sequence is one HIP graph, captured on the first call and replayed
+    by later identical calls.  Returns the timings dict (ms_total)."""
+    S, N = X.shape
+    G = set_offsets.numel() - 1
+    double = X.dtype.itemsize == 8
+    labels = np.ascontiguousarray(labels, dtype=np.uint8)
+    mids = np.ascontiguousarray([metric_id(m) for m in metrics], dtype=np.int32)
+    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
+        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    t = _lib.gsea_timings()
+    fn = lib().gsea_sweep_device_f64 if double else lib().gsea_sweep_device_f32
+    st = ctypes.c_void_p(stream) if stream else None
+    check(fn(_dp(X), N, S, _ptr(labels), _dp(set_offsets), _dp(set_genes), G, num_perms,
+             _ptr(mids), len(mids), flags, int(bool(graph)), _dp(stats_out), st,
+             ctypes.byref(t)))
+    return t.as_dict()
+
+
 def run_budget_device(X, labels, set_offsets, set_genes, num_perms, metric,
                       budget, stats_out, es_pg_out=None, extension=None,
                       ascending=False, swap_labels=False, stream=None):

@@ -143,6 +143,26 @@ int gsea_run_device_f64(const double *d_exprs, int64_t num_genes,
                         double *d_es_out, double *d_stats_out, void *stream,
                         gsea_timings *timings);
 
+/* Several metrics over the same device-resident inputs in one call (no
+ * counterpart in the reference, whose R entry point runs one metric per
+ * call): the class lists once, then per metric the whole path of
+ * gsea_run_device_*, statistics into d_stats [num_metrics][5][G].  With
+ * use_graph != 0 the sequence is captured into one HIP graph on the first
+ * call and replayed by later calls with the same inputs, sizes, labels,
+ * metrics, flags, output, stream, tuning and set list (the set CSR's contents
+ * are checked); any other call captures again.  Results equal
+ * gsea_run_device_*'s metric by metric.  timings: ms_total only. */
+int gsea_sweep_device_f32(const float *d_exprs, int64_t num_genes, int64_t num_samples,
+                          const uint8_t *labels, const int32_t *d_set_offsets,
+                          const int32_t *d_set_genes, int64_t num_sets, int64_t num_perms,
+                          const int *metric_ids, int num_metrics, int flags, int use_graph,
+                          float *d_stats_out, void *stream, gsea_timings *timings);
+int gsea_sweep_device_f64(const double *d_exprs, int64_t num_genes, int64_t num_samples,
+                          const uint8_t *labels, const int32_t *d_set_offsets,
+                          const int32_t *d_set_genes, int64_t num_sets, int64_t num_perms,
+                          const int *metric_ids, int num_metrics, int flags, int use_graph,
+                          double *d_stats_out, void *stream, gsea_timings *timings);
+
 /* Budgeted run (no counterpart in the reference; used by the learned
  * permutation allocation).  Set s is scored for permutations
  * 0..budget[s]-1 only; every budget divides num_perms and is >= 2.  Its
