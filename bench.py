@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4          # MI355X peak engine clock
 
 
 def algorithmic_bytes(kernel, N, S, G, P, w):
@@ -38,9 +39,39 @@ def algorithmic_bytes(kernel, N, S, G, P, w):
     return P * (N * (w + 4) + G * w) + N * G / 8
 
 
+def kernel_profile(kernel, config, perms):
+    """The committed counter summary of a kernel at this workload
+    (profiles/counters_<kernel>_<config>.txt, JSON), when its permutation
+    count matches the launch: HBM bytes per launch (FETCH_SIZE and
+    WRITE_SIZE, calibrated by a copy of known size), VALU instructions and
+    LDS occupancy per launch."""
+    path = os.path.join(ROOT, "profiles", "counters_%s_%s.txt" % (kernel, config))
+    if not os.path.exists(path):
+        return {}
+    try:
+        rec = json.load(open(path))
+    except (ValueError, OSError):
+        return {}
+    if int(rec.get("permutations", -1)) != int(perms):
+        return {}
+    rec["source"] = os.path.relpath(path, ROOT)
+    return rec
+
+
 def path_bytes_per_es(N, S, G, P, w):
     """Whole-path algorithmic bytes per enrichment score (all stages)."""
     return (P * (N * (4 * w + 8) + S + G * w) + N * S * w + N * G / 8) / (G * P)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(w, metric, target_s):
@@ -60,11 +91,50 @@ def cpu_baseline(w, metric, target_s):
     oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"], P1,
                 metric, want_es=False)
     dt = time.perf_counter() - t
-    return {"value": w["G"] * P1 / dt, "unit": "ES/s", "cores": cores,
-            "kind": "port",
-            "sample": "%s, %d genes x %d samples, %d sets, %d permutations "
-                      "(of %d), %.1f s on %d threads"
-                      % (metric, w["N"], w["S"], w["G"], P1, w["P"], dt, cores)}
+    out = {"value": w["G"] * P1 / dt, "unit": "ES/s", "cores": cores,
+           "kind": "port", "cpu": cpu_model(),
+           "sample": "%s, %d genes x %d samples, %d sets, %d permutations "
+                     "(of %d), %.1f s on %d threads"
+                     % (metric, w["N"], w["S"], w["G"], P1, w["P"], dt, cores)}
+    sizes = os.path.join(ROOT, "profiles", "cpu_baseline_sizes.txt")
+    if os.path.exists(sizes):
+        try:
+            out["sizes"] = json.load(open(sizes))
+        except ValueError:
+            pass
+    return out
+
+
+def cpu_sizes():
+    """The oracle at the configurations' own sizes: c1, c2 (fp32 and fp64)
+    and c4 in full, c3 on 1,024 of its permutations (stage cost is linear in
+    the permutation count).  Input generation is excluded, as the
+    reference's timers exclude parsing (cugsea.cuh:81)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from cudagsea_amd import workloads
+    oracle.build(ref=False)
+    cores = oracle.lib().orc_num_threads()
+    runs = [("c1", False, None, "stable_signal2noise"),
+            ("c2", False, None, "onepass_t_test"),
+            ("c2", True, None, "onepass_t_test"),
+            ("c4", False, None, "twopass_signal2noise"),
+            ("c3", False, 1024, "onepass_t_test")]
+    out = {"cpu": cpu_model(), "cores": cores, "kind": "port", "runs": []}
+    for cfg, dbl, perms, metric in runs:
+        w = workloads.make(cfg, dtype=np.float64 if dbl else np.float32, num_perms=perms)
+        print("cpu %s %s P=%d ..." % (cfg, "fp64" if dbl else "fp32", w["P"]),
+              file=sys.stderr, flush=True)
+        t = time.perf_counter()
+        oracle.gsea(w["X"], w["labels"], w["set_offsets"], w["set_genes"], w["P"],
+                    metric, double=dbl, want_es=False)
+        dt = time.perf_counter() - t
+        out["runs"].append({"config": cfg, "dtype": "f64" if dbl else "f32",
+                            "metric": metric, "genes": w["N"], "samples": w["S"],
+                            "sets": w["G"], "permutations": w["P"],
+                            "seconds": round(dt, 3),
+                            "es_per_s": round(w["G"] * w["P"] / dt, 1)})
+    return out
 
 
 def surrogate_leg(args):
@@ -106,11 +176,21 @@ def main():
     ap.add_argument("--double", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--retune", action="store_true",
+                    help="search the launch shapes again (and cache them)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate-guided vs plain comparison (c5)")
+    ap.add_argument("--cpu-sizes", action="store_true",
+                    help="only time the CPU oracle at the configurations' sizes")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES on the device (counter calibration)")
     args = ap.parse_args()
+
+    if args.cpu_sizes:
+        print(json.dumps(cpu_sizes()), flush=True)
+        return
 
     if args.probe_copy:
         # calibration for HBM byte counters: one device-to-device copy of a
@@ -133,7 +213,7 @@ def main():
 
     import torch
     import cudagsea_amd as cg
-    from cudagsea_amd import workloads
+    from cudagsea_amd import autotune, workloads
 
     torch.cuda.set_device(0)
     dt = np.float64 if args.double else np.float32
@@ -144,17 +224,40 @@ def main():
     X = torch.from_numpy(w["X"]).cuda()
     off = torch.from_numpy(w["set_offsets"]).cuda()
     genes = torch.from_numpy(w["set_genes"]).cuda()
-    stats = torch.empty((5, G), dtype=X.dtype, device="cuda")
+    stats = torch.empty((len(metrics), 5, G), dtype=X.dtype, device="cuda")
     print("bench: %s N=%d S=%d G=%d P=%d metrics=%d" % (args.config, N, S, G, P,
                                                          len(metrics)),
           file=sys.stderr, flush=True)
 
-    def step():
-        tims = []
-        for m in metrics:
-            tims.append(cg.run_device(X, w["labels"], off, genes, P, m, stats))
-        return tims
+    # launch shapes: the autotuner's cached choice for this workload and
+    # device, searched now (and cached) when there is none
+    knobs, source = {}, "defaults"
+    if not args.no_tune:
+        knobs = autotune.lookup(N, S, G, P, args.double, metrics)
+        source = "cache"
+        if knobs is None or args.retune:
+            print("autotune ...", file=sys.stderr, flush=True)
+            _, res = autotune.tune_workload(dict(w, metrics=metrics), args.double,
+                                            log=lambda m: print(m, file=sys.stderr,
+                                                                flush=True))
+            knobs, source = res["knobs"], "search"
 
+    def step():
+        # the whole sweep (every metric) is one HIP graph, replayed per step
+        return cg.sweep_device(X, w["labels"], off, genes, P, metrics, stats, graph=True)
+
+    def timed(nsteps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tims = [step() for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, tims
+
+    # the default shapes, for the tuned-vs-default comparison (untimed steps)
+    cg.set_tuning()
+    step()
+    default_s, _ = timed(1)
+    cg.set_tuning(**knobs)
     for i in range(args.warmup):
         step()
         print("warmup %d done" % i, file=sys.stderr, flush=True)
@@ -162,12 +265,12 @@ def main():
     t0 = time.perf_counter()
     all_t = []
     for i in range(args.steps):
-        all_t.extend(step())
+        all_t.append(step())
         print("step %d done" % i, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
-    n_runs = len(all_t)
+    n_runs = len(all_t) * len(metrics)
     es_total = float(G) * P * n_runs
     value = es_total / elapsed
     ms_metric = sum(t["ms_metric"] for t in all_t)
@@ -177,8 +280,9 @@ def main():
     batches = sum(t["perm_batches"] for t in all_t)
     es_launches = sum(t["es_launches"] for t in all_t)
     wbytes = np.dtype(dt).itemsize
-    # dominant kernel: its launches' average duration (HIP events on the
-    # engine's stream, inside the timed region) against its algorithmic bytes
+    # dominant kernel: its launches' average duration (HIP event nodes around
+    # each launch inside the replayed graph, in the timed region) against its
+    # algorithmic bytes
     if ms_es >= ms_metric:
         name, ms, launches = "k_es", ms_es, es_launches
     else:
@@ -187,27 +291,35 @@ def main():
     per_launch_P = P * n_runs / launches
     abytes = algorithmic_bytes(name, N, S, G, per_launch_P, wbytes)
     achieved = abytes / (per_launch_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (name, args.config))
-    if os.path.exists(tpath):
-        try:
-            rec = json.load(open(tpath))
-            if int(rec.get("permutations", -1)) == int(per_launch_P):
-                traffic = rec.get("bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
+    prof = kernel_profile(name, args.config, per_launch_P)
     bpe = path_bytes_per_es(N, S, G, P, wbytes)
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": traffic, "kernel": name,
+            "traffic": prof.get("bytes_per_launch"), "kernel": name,
             "launch_ms": round(per_launch_ms, 4),
             "algorithmic_bytes": int(abytes),
             "path_bytes_per_es": round(bpe, 3),
             "path_frac": round(bpe * value / (HBM_PEAK_GBS * 1e9), 6)}
+    if prof.get("valu_insts_per_launch"):
+        # the kernel's own bound: vector-instruction issue (one wavefront
+        # instruction per CU per clock) and LDS occupancy, from its counters
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        peak_issue = cus * CLOCK_GHZ * 1e9
+        issue = prof["valu_insts_per_launch"] / (per_launch_ms * 1e-3)
+        roof["issue"] = {"bound": "valu", "achieved_winst_per_s": round(issue, 1),
+                         "peak_winst_per_s": peak_issue,
+                         "frac": round(issue / peak_issue, 4),
+                         "lds_active_frac": prof.get("lds_active_frac"),
+                         "lds_bank_conflict_frac": prof.get("lds_bank_conflict_frac"),
+                         "profile": prof.get("source")}
     stage_ms = {"metric": round(ms_metric / n_runs, 3),
                 "rank": round(ms_rank / n_runs, 3),
                 "es": round(ms_es / n_runs, 3),
                 "stats": round(ms_stats / n_runs, 3)}
+    tuning = {"knobs": knobs, "source": source,
+              "default_ms_per_step": round(default_s * 1e3, 2),
+              "tuned_ms_per_step": round(elapsed * 1e3 / args.steps, 2),
+              "graph": "one HIP graph per step (%d metrics), replayed" % len(metrics)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -231,7 +343,7 @@ def main():
                    "genes": N, "samples": S, "gene_sets": G,
                    "permutations": P, "metrics": len(metrics),
                    "mean_ms_per_metric_run": stage_ms},
-        "roofline": roof, "cpu_baseline": cpu, "surrogate": sur,
+        "roofline": roof, "tuning": tuning, "cpu_baseline": cpu, "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 

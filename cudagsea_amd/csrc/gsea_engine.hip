@@ -68,7 +68,7 @@ enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
     kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp,
-    kBufFdrTable, kBufFdrInfo, kBufCheck, kNumBufs
+    kBufFdrTable, kBufFdrInfo, kBufCheck, kBufStamps, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -154,14 +154,30 @@ static int ensure(DeviceState *d, BufId id, size_t bytes, void **out) {
 }
 
 // events recorded around launches; durations summed per stage after sync
+// device wall clock at a point of a stream (one thread, after the previous
+// launch on the stream completes and before the next starts)
+__global__ void k_stamp(unsigned long long *slot) { *slot = (unsigned long long)wall_clock64(); }
+
+// Stage boundaries of a run: HIP events, or -- for sequences captured into a
+// graph, where event nodes do not time -- device wall-clock stamps written
+// by k_stamp into `stamps` (read after the stream is synchronised).
 struct StageClock {
     DeviceState *d;
     hipStream_t st;
     std::vector<std::pair<int, std::pair<int, int>>> marks;  // stage, ev0, ev1
     int next = 0;
     int open_stage = -1, open_ev = -1;
+    unsigned long long *stamps = nullptr;   // device; stamp mode when set
+    int stamp_cap = 0;
 
     int event(int *idx) {
+        if (stamps) {
+            if (next >= stamp_cap) return fail(GSEA_ERR_INVALID_ARGUMENT, "too many stages");
+            *idx = next++;
+            hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, st, stamps + *idx);
+            HIP_TRY(hipGetLastError());
+            return GSEA_OK;
+        }
         if (next >= (int)d->events.size()) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
@@ -189,20 +205,40 @@ static int finish_timings(StageClock &clk, gsea_timings *t, int64_t batches) {
     std::memset(t, 0, sizeof(*t));
     t->perm_batches = batches;
     if (clk.marks.empty()) return GSEA_OK;
+    std::vector<unsigned long long> ts;
+    double ms_per_tick = 0.0;
+    if (clk.stamps) {
+        int dev = 0, khz = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) return fail(GSEA_ERR_HIP, "no wall clock rate");
+        ms_per_tick = 1.0 / (double)khz;
+        ts.resize((size_t)clk.next);
+        HIP_TRY(hipMemcpy(ts.data(), clk.stamps, sizeof(unsigned long long) * ts.size(),
+                          hipMemcpyDeviceToHost));
+    }
+    auto span = [&](int a, int b, float *ms) -> int {
+        if (clk.stamps) {
+            *ms = (float)((double)(long long)(ts[(size_t)b] - ts[(size_t)a]) * ms_per_tick);
+            return GSEA_OK;
+        }
+        HIP_TRY(hipEventElapsedTime(ms, clk.d->events[a], clk.d->events[b]));
+        return GSEA_OK;
+    };
     // stages: 0 lists, 1 metric, 2 rank, 3 statistics, 4 transpose, 5 ES
     double *slots[6] = {&t->ms_lists, &t->ms_metric, &t->ms_rank,
                         &t->ms_stats, &t->ms_transpose, &t->ms_es};
+    int rc;
     for (auto &m : clk.marks) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, clk.d->events[m.second.first],
-                                    clk.d->events[m.second.second]));
+        if ((rc = span(m.second.first, m.second.second, &ms))) return rc;
         *slots[m.first] += ms;
         if (m.first == 5) ++t->es_launches;
     }
     t->ms_rank_es = t->ms_rank + t->ms_es;
     float tot = 0.f;
-    HIP_TRY(hipEventElapsedTime(&tot, clk.d->events[clk.marks.front().second.first],
-                                clk.d->events[clk.marks.back().second.second]));
+    if ((rc = span(clk.marks.front().second.first, clk.marks.back().second.second, &tot)))
+        return rc;
     t->ms_total = tot;
     return GSEA_OK;
 }
@@ -1020,6 +1056,10 @@ struct SweepGraph {
     SweepKey key;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // stage events recorded by the graph's event nodes (stage, ev0, ev1)
+    std::vector<std::pair<int, std::pair<int, int>>> marks;
+    int nstamps = 0;
+    int64_t batches = 0;
 };
 
 static SweepGraph g_sweep[64];
@@ -1065,66 +1105,75 @@ static int run_sweep(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
     if (rc) return rc;
     const bool one_batch = c.Pb >= P;
     const bool capturable = use_graph && c.sb.npow2 > 0;
+    // stage events bracket every launch; in a graph they become event nodes
+    StageClock clk{d, st};
+    int64_t batches = 0;
     auto enqueue_all = [&]() -> int {
         int r;
-        if (one_batch && (r = enqueue_lists<T>(c, 0, P, st))) return r;
+        if (one_batch) {
+            if ((r = clk.begin(0))) return r;
+            if ((r = enqueue_lists<T>(c, 0, P, st))) return r;
+            if ((r = clk.end())) return r;
+        }
         for (int m = 0; m < nm; ++m) {
-            r = enqueue_run<T>(c, metrics[m], d_stats + (size_t)m * 5 * G, nullptr, st, nullptr,
-                               one_batch, nullptr);
+            int64_t nb = 0;
+            r = enqueue_run<T>(c, metrics[m], d_stats + (size_t)m * 5 * G, nullptr, st, &clk,
+                               one_batch, &nb);
             if (r) return r;
+            batches += nb;
         }
         return GSEA_OK;
     };
-    hipEvent_t e0, e1;
-    StageClock clk{d, st};
-    int i0, i1;
-    if ((rc = clk.event(&i0))) return rc;
-    e0 = d->events[i0];
     if (!capturable) {
         if ((rc = enqueue_all())) return rc;
-    } else {
-        SweepKey k;
-        k.dX = dX; k.N = N; k.S = S; k.G = G; k.P = P;
-        k.labels.assign(labels, labels + S);
-        k.off = d_off; k.genes = d_genes;
-        k.metrics.assign(metrics, metrics + nm);
-        k.flags = flags; k.stats = d_stats; k.st = st; k.tsize = sizeof(T);
-        k.tune = g_tune;
-        k.tiers = d->tier_key;
-        k.generation = g_ws_generation;
-        SweepGraph &sg = g_sweep[dev];
-        if (!(sg.valid && sg.key == k)) {
-            drop_graph(sg);
-            // the event above is recorded outside the capture
-            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-            rc = enqueue_all();
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(st, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (ec != hipSuccess)
-                return fail(GSEA_ERR_HIP, std::string("hipStreamEndCapture: ") +
-                                              hipGetErrorString(ec));
-            sg.graph = g;
-            HIP_TRY(hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0));
-            sg.key = k;
-            sg.valid = g_ws_generation == k.generation;
+        HIP_TRY(hipStreamSynchronize(st));
+        return finish_timings(clk, t, batches);
+    }
+    SweepKey k;
+    k.dX = dX; k.N = N; k.S = S; k.G = G; k.P = P;
+    k.labels.assign(labels, labels + S);
+    k.off = d_off; k.genes = d_genes;
+    k.metrics.assign(metrics, metrics + nm);
+    k.flags = flags; k.stats = d_stats; k.st = st; k.tsize = sizeof(T);
+    k.tune = g_tune;
+    k.tiers = d->tier_key;
+    SweepGraph &sg = g_sweep[dev];
+    // stage stamps of the graph: two per stage launch
+    const int cap = 2 * (1 + nm * (int)(4 * ((P + c.Pb - 1) / c.Pb) + 2));
+    void *pstamp;
+    if ((rc = ensure(d, kBufStamps, sizeof(unsigned long long) * cap, &pstamp))) return rc;
+    k.generation = g_ws_generation;
+    if (!(sg.valid && sg.key == k)) {
+        drop_graph(sg);
+        clk.stamps = (unsigned long long *)pstamp;
+        clk.stamp_cap = cap;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        rc = enqueue_all();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
         }
-        HIP_TRY(hipGraphLaunch(sg.exec, st));
+        if (ec != hipSuccess)
+            return fail(GSEA_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                          hipGetErrorString(ec));
+        sg.graph = g;
+        HIP_TRY(hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0));
+        sg.key = k;
+        sg.marks = clk.marks;
+        sg.nstamps = clk.next;
+        sg.batches = batches;
+        sg.valid = g_ws_generation == k.generation;
     }
-    if ((rc = clk.event(&i1))) return rc;
-    e1 = d->events[i1];
+    HIP_TRY(hipGraphLaunch(sg.exec, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (t) {
-        std::memset(t, 0, sizeof(*t));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        t->ms_total = ms;
-        t->perm_batches = (P + c.Pb - 1) / c.Pb;
-    }
-    return GSEA_OK;
+    StageClock done{d, st};
+    done.marks = sg.marks;
+    done.stamps = (unsigned long long *)pstamp;
+    done.stamp_cap = cap;
+    done.next = sg.nstamps;
+    return finish_timings(done, t, sg.batches);
 }
 
 // ---------------------------------------------------------------------------

@@ -105,3 +105,22 @@ def test_statistics_host_threshold_fallback_matches_oracle(gpu):
                                             "stable_signal2noise")
     got = gpu.run(w["X"], w["labels"], off, genes, P, "stable_signal2noise")
     parity.assert_stats(got, want, es_want, hi, lo)
+
+
+def test_sweep_graph_stage_times_agree_with_events(gpu):
+    # the graph's stage times come from device wall-clock stamps around each
+    # launch; eager runs time the same stages with HIP events
+    import torch
+    w = workloads.make("c3", num_genes=4000, num_sets=600, num_perms=1024)
+    P, G = 1024, w["G"]
+    X = dev(w["X"])
+    off, genes = dev(w["set_offsets"]), dev(w["set_genes"])
+    st = torch.empty((2, 5, G), dtype=torch.float32, device="cuda")
+    gpu.sweep_device(X, w["labels"], off, genes, P, METRICS4[:2], st, graph=True)
+    tg = gpu.sweep_device(X, w["labels"], off, genes, P, METRICS4[:2], st, graph=True)
+    te = gpu.sweep_device(X, w["labels"], off, genes, P, METRICS4[:2], st, graph=False)
+    assert tg["es_launches"] == te["es_launches"] == 2
+    for k in ("ms_metric", "ms_es", "ms_stats"):
+        assert tg[k] > 0 and te[k] > 0, k
+        assert abs(tg[k] - te[k]) <= 0.25 * te[k] + 0.05, (k, tg[k], te[k])
+    assert tg["ms_total"] >= tg["ms_metric"] + tg["ms_es"]
