@@ -68,7 +68,7 @@ enum BufId {
     kBufLabels, kBufSig, kBufLists, kBufR, kBufES, kBufAGlob, kBufFactor,
     kBufMax, kBufMin, kBufThr, kBufHist, kBufX, kBufOff, kBufGenes,
     kBufStats, kBufEsOut, kBufPos, kBufBudget, kBufCv, kBufRcp,
-    kBufFdrTable, kBufFdrInfo, kBufCheck, kBufStamps, kNumBufs
+    kBufFdrTable, kBufFdrInfo, kBufCheck, kBufStamps, kBufRankScratch, kNumBufs
 };
 
 // set tiers of one set list (slot 0: all sets; slots 1..: budget levels)
@@ -350,6 +350,8 @@ struct LdsPlan {
 };
 
 struct RankPlan {
+    bool big = false;  // k_rank_big: more genes than k_rank's LDS sort holds
+    int npad = 0;      // k_rank_big: ids per wavefront segment x 16, multiple of 1024
     int maxch = 1;   // k_rank: chunks of 64 keys per wavefront
     int W = 1;       // k_es: bitmap words per lane
     int nw = 0;      // k_es: bitmap words per wavefront (64 W)
@@ -371,8 +373,9 @@ static bool fit(size_t region0, size_t npad, size_t region2, int lds_max, LdsPla
 template <class T>
 static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     pl->maxch = pick_maxch(N);
-    if (pl->maxch == 0)
-        return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
+    pl->big = pl->maxch == 0;
+    pl->npad = (int)((N + kRankThreads - 1) / kRankThreads * kRankThreads);
+    if (pl->big) pl->maxch = 32;   // k_rank's LDS fields below are unused
     const size_t npad = (size_t)kRankThreads * pl->maxch;
     // k_es's bitmap holds ranks 0..N: rank N is the sentinel's (|metric| 0,
     // bit cleared before the walk), so N + 1 bits, W words per lane
@@ -381,7 +384,7 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     const size_t counters = (size_t)kRankWaves * 256 * 4;
     // k_rank keeps fp32 keys in LDS; fp64 keys are recomputed from r
     pl->rank.alds = sizeof(T) == 4 && fit(npad * 4, npad, counters, d->lds_max, &pl->rank);
-    if (!pl->rank.alds && !fit(0, npad, counters, d->lds_max, &pl->rank))
+    if (!pl->big && !pl->rank.alds && !fit(0, npad, counters, d->lds_max, &pl->rank))
         return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the rank kernel");
     // k_es: |metric| by rank and pos for genes/ranks 0..N (the sentinel
     // included), not padded to the rank kernel's chunks
@@ -634,6 +637,9 @@ static int launch_es_rows(int64_t N, int64_t rows_cap, int64_t perm_base,
     return GSEA_OK;
 }
 
+// workgroups of k_rank_big: two per CU, each sorting its rows in turn
+static int64_t rank_big_grid(const DeviceState *d) { return 2LL * d->cu_count; }
+
 // k_rank over the Pb rows of a batch (rank rows kept for k_es), then k_es of
 // every row for the sets of `tiers` (none: ranks only)
 template <class T>
@@ -657,11 +663,19 @@ static int launch_rank(const T *r, int64_t N, int64_t Pb, int64_t perm_base,
         a.a_glob = (T *)pa;
         a.pos_out = (uint16_t *)pp;
     }
-    const int grid = (int)Pb;
     a.es_waves = kRankWaves;
-    a.off_ids = pl.rank.off_ids; a.off_cnt = pl.rank.off_cnt; a.off_misc = pl.rank.off_misc;
-    hipError_t e = launch_rank_kernel<T>(a, pl.rank.alds, pl.maxch, pl.rank.lds, grid, st);
-    HIP_TRY(e);
+    if (pl.big) {
+        // ids ping-pong through 2 x npad entries per workgroup
+        const int grid = (int)std::min<int64_t>(Pb, rank_big_grid(d));
+        void *ps;
+        rc = ensure(d, kBufRankScratch, (size_t)grid * 2 * pl.npad * sizeof(uint16_t), &ps);
+        if (rc) return rc;
+        HIP_TRY(launch_rank_big_kernel<T>(a, (uint16_t *)ps, pl.npad, grid, st));
+    } else {
+        const int grid = (int)Pb;
+        a.off_ids = pl.rank.off_ids; a.off_cnt = pl.rank.off_cnt; a.off_misc = pl.rank.off_misc;
+        HIP_TRY(launch_rank_kernel<T>(a, pl.rank.alds, pl.maxch, pl.rank.lds, grid, st));
+    }
     if (G > 0)
         return launch_es_rows<T>(N, Pb, perm_base, d_off, d_genes, G, flags, es_pg, d, st,
                                  tiers, 0, Pb);
@@ -936,11 +950,19 @@ static int prepare_run(const T *dX, int64_t N, int64_t S, const uint8_t *labels,
     rc = cached_tiers<T>(d_off, d_genes, G, N, d, st, &c->tiers);
     if (!rc) rc = prepare_stats<T>(P, G, d, &c->sb);
     if (rc) return rc;
-    // rank rows handed from k_rank to k_es
+    // rank rows handed from k_rank to k_es, and k_rank_big's id scratch
     void *pa, *pp;
     rc = ensure(d, kBufAGlob, (size_t)c->Pb * N * sizeof(T), &pa);
     if (!rc) rc = ensure(d, kBufPos, (size_t)c->Pb * N * sizeof(uint16_t), &pp);
     if (rc) return rc;
+    RankPlan rp;
+    if ((rc = plan_rank<T>(N, d, &rp))) return rc;
+    if (rp.big) {
+        void *ps;
+        const int64_t grid = std::min<int64_t>(c->Pb, rank_big_grid(d));
+        rc = ensure(d, kBufRankScratch, (size_t)grid * 2 * rp.npad * sizeof(uint16_t), &ps);
+        if (rc) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(st));   // the labels' host copy
     return GSEA_OK;
 }

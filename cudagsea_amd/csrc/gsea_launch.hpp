@@ -23,6 +23,12 @@ template <class T>
 hipError_t launch_rank_kernel(const RankArgs<T> &a, bool alds, int maxch, size_t lds,
                               int grid, hipStream_t st);
 
+// k_rank_big (gene counts beyond k_rank's LDS sort): scratch holds
+// 2 * npad 16-bit ids per workgroup of the grid
+template <class T>
+hipError_t launch_rank_big_kernel(const RankArgs<T> &a, uint16_t *scratch, int npad, int grid,
+                                  hipStream_t st);
+
 // k_es, |metric| by rank in LDS when alds (fp32 only)
 template <class T>
 hipError_t launch_es_kernel(const RankArgs<T> &a, bool alds, size_t lds, int grid,
@@ -37,6 +43,10 @@ template <>
 hipError_t launch_es_kernel<float>(const RankArgs<float> &, bool, size_t, int, hipStream_t);
 template <>
 hipError_t launch_es_kernel<double>(const RankArgs<double> &, bool, size_t, int, hipStream_t);
+extern template hipError_t launch_rank_big_kernel<float>(const RankArgs<float> &, uint16_t *,
+                                                         int, int, hipStream_t);
+extern template hipError_t launch_rank_big_kernel<double>(const RankArgs<double> &, uint16_t *,
+                                                          int, int, hipStream_t);
 extern template hipError_t launch_rank_kernel<float>(const RankArgs<float> &, bool, int,
                                                      size_t, int, hipStream_t);
 extern template hipError_t launch_rank_kernel<double>(const RankArgs<double> &, bool, int,

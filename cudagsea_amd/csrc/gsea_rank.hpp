@@ -667,6 +667,128 @@ __global__ __launch_bounds__(kRankThreads) void k_rank(RankArgs<T> a) {
 }
 
 // ===========================================================================
+// k_rank_big: the rank rows of k_rank for gene counts beyond one workgroup's
+// LDS sort (N > 32,768; 16-bit ids, N <= 65,535).  The same stable LSD radix
+// passes with 8-bit digits and ballot-ranked local ranks, but the ids
+// ping-pong through a global scratch of 2 x Npad entries per workgroup, and
+// each pass sweeps the wavefront's chunks twice -- counting digits, then,
+// after the scan over (digit, wavefront), scattering -- instead of holding
+// every chunk in registers.  Keys are read from the metric row (L2).
+// ===========================================================================
+template <class T>
+__global__ __launch_bounds__(kRankThreads) void k_rank_big(RankArgs<T> a,
+                                                           uint16_t *__restrict__ scratch,
+                                                           int Npad) {
+    typedef typename KeyOf<T>::type KeyT;
+    constexpr int NPASS = (int)sizeof(KeyT);
+    __shared__ uint32_t cnt[kRankWaves * 256];
+    __shared__ int misc[kRankWaves];
+    const int N = a.N;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const bool asc = a.asc != 0;
+    const int seg = Npad / kRankWaves;          // a multiple of 64
+    const int CH = seg / kWave;
+    const KeyT kSentinel = ~(KeyT)0;
+    uint16_t *bufA = scratch + (size_t)blockIdx.x * 2 * Npad;
+    uint16_t *bufB = bufA + Npad;
+
+    for (int p = blockIdx.x; p < a.Pb; p += gridDim.x) {
+        const T *rp = a.r + (size_t)p * N;
+        auto key_of = [&](int g) -> KeyT {
+            return g < N ? (KeyT)sort_key(rp[g], asc) : kSentinel;
+        };
+        for (int g = tid; g < Npad; g += kRankThreads) bufA[g] = (uint16_t)g;
+        __syncthreads();
+        uint16_t *src = bufA, *dst = bufB;
+        const int start = w * seg;
+        uint32_t *wcnt = cnt + w * 256;
+        for (int pass = 0; pass < NPASS; ++pass) {
+            const int shift = pass * 8;
+            for (int i = lane; i < 256; i += kWave) wcnt[i] = 0;
+            wave_sync_lds();
+            // sweep 1: digit counts of the wavefront's segment
+            for (int c = 0; c < CH; ++c) {
+                const uint32_t id = src[start + c * kWave + lane];
+                const uint32_t d = (uint32_t)((key_of((int)id) >> shift) & 255u);
+                uint64_t peers = ~0ull;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t bal = ballot64(bit);
+                    peers &= bit ? bal : ~bal;
+                }
+                if (mbcnt64(peers) == 0) wcnt[d] += (uint32_t)__popcll(peers);
+                wave_sync_lds();
+            }
+            __syncthreads();
+            // exclusive scan over (digit, wavefront), digit-major
+            uint32_t tot_d = 0;
+            if (tid < 256) {
+                uint32_t run = 0;
+                for (int w2 = 0; w2 < kRankWaves; ++w2) {
+                    const uint32_t v = cnt[w2 * 256 + tid];
+                    cnt[w2 * 256 + tid] = run;
+                    run += v;
+                }
+                tot_d = run;
+                uint32_t inc = tot_d;
+#pragma unroll
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const uint32_t o = __shfl_up(inc, off, kWave);
+                    if (lane >= off) inc += o;
+                }
+                if (lane == kWave - 1) misc[w] = (int)inc;
+                tot_d = inc - tot_d;
+            }
+            __syncthreads();
+            if (tid < 256) {
+                uint32_t base = tot_d;
+                for (int w2 = 0; w2 < w; ++w2) base += (uint32_t)misc[w2];
+                for (int w2 = 0; w2 < kRankWaves; ++w2) cnt[w2 * 256 + tid] += base;
+            }
+            __syncthreads();
+            // sweep 2: the same chunks in the same order, scattered
+            for (int c = 0; c < CH; ++c) {
+                const uint32_t id = src[start + c * kWave + lane];
+                const uint32_t d = (uint32_t)((key_of((int)id) >> shift) & 255u);
+                uint64_t peers = ~0ull;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t bal = ballot64(bit);
+                    peers &= bit ? bal : ~bal;
+                }
+                const int before = mbcnt64(peers);
+                const uint32_t base = wcnt[d];
+                dst[base + (uint32_t)before] = (uint16_t)id;
+                wave_sync_lds();
+                if (before == 0) wcnt[d] = base + (uint32_t)__popcll(peers);
+                wave_sync_lds();
+            }
+            __syncthreads();
+            uint16_t *t = src;
+            src = dst;
+            dst = t;
+        }
+        // src holds the gene at each rank (padding ranks >= N last)
+        if (a.idx_out)
+            for (int i = tid; i < N; i += kRankThreads)
+                a.idx_out[(size_t)p * N + i] = src[i];
+        if (a.pos_out) {
+            T *arow = a.a_glob + (size_t)p * N;
+            for (int i = tid; i < N; i += kRankThreads) {
+                const int g = src[i];
+                arow[i] = (T)__builtin_fabs((double)rp[g]);
+                a.pos_out[(size_t)p * N + g] = (uint16_t)i;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
 // k_es: enrichment scores of every gene set for one permutation per
 // workgroup, from the rank rows k_rank wrote (pos[gene], |metric| by rank).
 // Both rows are staged in LDS (|metric| only in fp32; fp64 reads it from L2).

@@ -40,6 +40,17 @@ hipError_t launch_rank_kernel(const RankArgs<T> &a, bool alds, int maxch, size_t
                 : launch_rank_alds<T, false>(a, maxch, lds, grid, st);
 }
 
+template <class T>
+hipError_t launch_rank_big_kernel(const RankArgs<T> &a, uint16_t *scratch, int npad, int grid,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(k_rank_big<T>, dim3(grid), dim3(kRankThreads), 0, st, a, scratch, npad);
+    return hipGetLastError();
+}
+
+template hipError_t launch_rank_big_kernel<float>(const RankArgs<float> &, uint16_t *, int, int,
+                                                  hipStream_t);
+template hipError_t launch_rank_big_kernel<double>(const RankArgs<double> &, uint16_t *, int,
+                                                   int, hipStream_t);
 template hipError_t launch_rank_kernel<float>(const RankArgs<float> &, bool, int, size_t,
                                               int, hipStream_t);
 template hipError_t launch_rank_kernel<double>(const RankArgs<double> &, bool, int, size_t,

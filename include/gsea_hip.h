@@ -48,7 +48,7 @@ extern "C" {
 #define GSEA_FLAG_SWAP_LABELS 2   /* reference swap_labels == true */
 
 /* engine limits */
-#define GSEA_MAX_GENES   32768
+#define GSEA_MAX_GENES   65535   /* 16-bit rank rows */
 #define GSEA_MAX_SAMPLES 65535
 
 /* per-stage device times of the last call, milliseconds (HIP events) */

@@ -92,7 +92,7 @@ def test_argument_errors_before_any_device_call():
         cg.run(X, lab, off, np.array([1, 10], dtype=np.int32), 8, 0)
     assert e.value.code == 30            # gene out of range
     with pytest.raises(cg.GseaError) as e:
-        cg.run(np.ones((4, 40000), dtype=np.float32), lab, off, genes, 8, 0)
+        cg.run(np.ones((4, 65536), dtype=np.float32), lab, off, genes, 8, 0)
     assert e.value.code == 31            # above GSEA_MAX_GENES
 
 

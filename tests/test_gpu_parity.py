@@ -414,3 +414,58 @@ def test_multi_batch_bit_identical(gpu):
         assert t1["perm_batches"] == 1 and t2["perm_batches"] == 3
         assert bits_equal(st1.cpu().numpy(), st2.cpu().numpy()).all()
         assert bits_equal(es1.cpu().numpy(), es2.cpu().numpy()).all()
+
+
+# --------------------------------------------- beyond one LDS sort ----
+@pytest.mark.parametrize("dbl", [False, True])
+@pytest.mark.parametrize("N", [40000, 60000])
+def test_large_gene_counts_rank_and_es(gpu, N, dbl):
+    # more genes than one workgroup's LDS sort (32,768): the rank rows come
+    # from the global-scratch radix passes, the ES kernel reads |metric|
+    # from L2; ranks bit-exact, ES to the bar, with small and large sets
+    import torch
+    print("large N", N, "fp64" if dbl else "fp32", flush=True)
+    rng = np.random.default_rng(N + dbl)
+    dt = np.float64 if dbl else np.float32
+    P = 6
+    keys = rng.standard_normal((P, N)).astype(dt)
+    keys[0, :2000] = 0.0                                # ties
+    keys[1, ::101] = np.nan
+    _, want_idx = oracle.pathway_sort(keys)
+    idx = torch.empty((P, N), dtype=torch.int32, device="cuda")
+    gpu.argsort_device(dev(keys), idx)
+    assert (idx.cpu().numpy().view(np.uint32) == want_idx).all()
+    w = small("c1", num_genes=N, num_perms=P, num_sets=2)
+    sets = [np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
+            for k in (1, 7, 60, 128, 129, 500, 2000, 9000)]
+    sets += [np.array([0, N - 1], np.int32), np.arange(N - 300, N, dtype=np.int32)]
+    w["set_offsets"], w["set_genes"] = csr(sets)
+    w["G"] = len(sets)
+    got, want, hi, lo = _es_stage_case(gpu, w, P, "stable_signal2noise", dbl)
+    parity.assert_es(got, want, hi, lo)
+
+
+def test_large_gene_count_full_path(gpu):
+    print("full path, 40,000 genes", flush=True)
+    w = small("c1", num_genes=40000, num_perms=64, num_sets=40)
+    want, es_want, hi, lo = oracle.gsea_ext(w["X"], w["labels"], w["set_offsets"],
+                                            w["set_genes"], 64, "stable_signal2noise")
+    got, es_got = gpu.run(w["X"], w["labels"], w["set_offsets"], w["set_genes"], 64,
+                          "stable_signal2noise", want_es=True)
+    parity.assert_es(es_got, es_want, hi, lo)
+    parity.assert_stats(got, want, es_want, hi, lo)
+
+
+def test_gene_limit_is_65535(gpu):
+    import torch
+    rng = np.random.default_rng(5)
+    keys = rng.standard_normal((2, 65535)).astype(np.float32)
+    _, want = oracle.pathway_sort(keys)
+    idx = torch.empty((2, 65535), dtype=torch.int32, device="cuda")
+    gpu.argsort_device(dev(keys), idx)
+    assert (idx.cpu().numpy().view(np.uint32) == want).all()
+    keys2 = np.zeros((1, 65536), np.float32)
+    idx2 = torch.empty((1, 65536), dtype=torch.int32, device="cuda")
+    with pytest.raises(gpu.GseaError) as e:
+        gpu.argsort_device(dev(keys2), idx2)
+    assert e.value.code == 31
