@@ -16,6 +16,7 @@
 #include <hip/hip_bf16.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -374,7 +375,13 @@ struct MlpWork {
     float *buf = nullptr;
     size_t cap = 0;   // floats
 };
+// Scratch of gsea_mlp_grad (g_work) and gsea_set_features (g_pos), shared
+// by every caller in the process.  Both entry points hold g_sg_mu from the
+// first use of their scratch until their stream has drained, so a buffer is
+// never regrown or reused while a kernel of another call still reads it,
+// whichever threads or streams the calls come from.
 static MlpWork g_work, g_pos;
+static std::mutex g_sg_mu;
 
 static int scratch(int B, MlpScratch *sc, float **loss_part, float **grad) {
     const size_t need = (size_t)B * (6 * kH + kOut) + (size_t)(B / 256 + 2) + kNumParams;
@@ -431,6 +438,7 @@ int gsea_mlp_grad(const float *d_params, const float *d_x, const float *d_t,
     if (B <= 0 || B > (1LL << 26) || !d_params || !d_x || !d_t || !d_grad)
         return set_error(GSEA_ERR_INVALID_ARGUMENT, "bad mlp grad arguments");
     hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_sg_mu);
     MlpScratch sc;
     float *loss_part, *grad;
     int rc = scratch((int)B, &sc, &loss_part, &grad);
@@ -476,6 +484,7 @@ int gsea_set_features(const float *d_r0, const int32_t *d_rank_genes, int64_t N,
         !d_glob)
         return set_error(GSEA_ERR_INVALID_ARGUMENT, "bad feature arguments");
     hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_sg_mu);
     if (g_pos.cap < (size_t)N) {
         if (g_pos.buf) SG_TRY(hipFree(g_pos.buf));
         g_pos.buf = nullptr;
