@@ -26,6 +26,7 @@ GSEA_ERR_OUT_OF_MEMORY = 33
 
 FLAG_ASCENDING = 1
 FLAG_SWAP_LABELS = 2
+FLAG_SIGNED_NES = 4
 
 MAX_GENES = 65535
 MAX_SAMPLES = 65535

@@ -702,7 +702,7 @@ static int64_t hist_len(int64_t G) { return 4 * (G + 1) + 2; }
 template <class T>
 static int prepare_stats(int64_t P, int64_t G, DeviceState *d, StatsBufs<T> *b) {
     void *pf, *pmax, *pmin, *pthr, *phist, *ptab, *pinfo;
-    int rc = ensure(d, kBufFactor, (size_t)G * sizeof(T), &pf);
+    int rc = ensure(d, kBufFactor, 2 * (size_t)G * sizeof(T), &pf);
     if (!rc) rc = ensure(d, kBufMax, (size_t)P * sizeof(T), &pmax);
     if (!rc) rc = ensure(d, kBufMin, (size_t)P * sizeof(T), &pmin);
     if (!rc) rc = ensure(d, kBufThr, (size_t)G * sizeof(T), &pthr);
@@ -782,15 +782,21 @@ static int host_thresholds(const T *d_stats, int64_t G, const StatsBufs<T> &b,
 // With the device threshold sort (b.npow2 > 0) nothing here waits for the
 // device or reads device memory on the host, so the sequence can be captured
 // into a graph.  budget (optional): set s uses permutations 0..budget[s]-1.
+// flags: GSEA_FLAG_SIGNED_NES selects the per-sign normalisation.
 template <class T>
 static int enqueue_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
                          const StatsBufs<T> &b, const DeviceState *d, hipStream_t st,
-                         const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
+                         int flags, const int32_t *d_budget = nullptr,
+                         float *d_cv = nullptr) {
     const int g = (int)G;
     // one wavefront per 64 sets: the sequential per-set sums are latency
     // bound, so spread them over as many CUs as possible
-    hipLaunchKernelGGL(k_set_stats<T>, dim3((g + 63) / 64), dim3(64), 0, st,
-                       es_pg, P, g, d_stats, b.factor, d_budget, d_cv);
+    if (flags & GSEA_FLAG_SIGNED_NES)
+        hipLaunchKernelGGL((k_set_stats<T, true>), dim3((g + 63) / 64), dim3(64), 0, st,
+                           es_pg, P, g, d_stats, b.factor, d_budget, d_cv);
+    else
+        hipLaunchKernelGGL((k_set_stats<T, false>), dim3((g + 63) / 64), dim3(64), 0, st,
+                           es_pg, P, g, d_stats, b.factor, d_budget, d_cv);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_perm_extrema<T>, dim3((unsigned)((P + 3) / 4)), dim3(256), 0,
                        st, es_pg, P, g, b.factor, d_budget, b.maxv, b.minv);
@@ -858,11 +864,11 @@ static int enqueue_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
 
 template <class T>
 static int run_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
-                     DeviceState *d, hipStream_t st,
+                     DeviceState *d, hipStream_t st, int flags,
                      const int32_t *d_budget = nullptr, float *d_cv = nullptr) {
     StatsBufs<T> b;
     int rc = prepare_stats<T>(P, G, d, &b);
-    if (!rc) rc = enqueue_stats<T>(es_pg, P, G, d_stats, b, d, st, d_budget, d_cv);
+    if (!rc) rc = enqueue_stats<T>(es_pg, P, G, d_stats, b, d, st, flags, d_budget, d_cv);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     return GSEA_OK;
@@ -1012,7 +1018,7 @@ static int enqueue_run(const RunCtx<T> &c, int metric, T *d_stats, T *d_es_out,
     }
     if (batches) *batches = nb;
     if (clk && (rc = clk->begin(3))) return rc;
-    rc = enqueue_stats<T>((const T *)c.pes, c.P, c.G, d_stats, c.sb, c.d, st);
+    rc = enqueue_stats<T>((const T *)c.pes, c.P, c.G, d_stats, c.sb, c.d, st, c.flags);
     if (rc) return rc;
     if (clk && (rc = clk->end())) return rc;
     if (d_es_out) {
@@ -1359,7 +1365,7 @@ static int run_device_budget(const T *dX, int64_t N, int64_t S, const uint8_t *l
     }
     HIP_TRY(hipMemcpyAsync(pbud, B.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
     if ((rc = clk.begin(3))) return rc;
-    rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, (const int32_t *)pbud,
+    rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, flags, (const int32_t *)pbud,
                       (float *)pcv);
     if (rc) return rc;
     if ((rc = clk.end())) return rc;
@@ -1394,7 +1400,7 @@ static int run_device_budget(const T *dX, int64_t N, int64_t S, const uint8_t *l
             }
             HIP_TRY(hipMemcpyAsync(pbud, B.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
             if ((rc = clk.begin(3))) return rc;
-            rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, (const int32_t *)pbud,
+            rc = run_stats<T>((const T *)pes, P, G, d_stats, d, st, flags, (const int32_t *)pbud,
                               (float *)pcv);
             if (rc) return rc;
             if ((rc = clk.end())) return rc;

@@ -127,7 +127,14 @@ __global__ __launch_bounds__(1024) void k_csr_checksum(const int32_t *__restrict
     }
 }
 
-template <class T>
+// Per set: ES, NES, NP (statistics.cuh:56-78) and the normalisation
+// factors, factor[s] for scores >= 0 and factor[G + s] for scores < 0.
+// SIGNED=false: the reference's one factor B / sum_p |ES_p| over permutations
+// 0..B-1 (both entries equal).  SIGNED=true (GSEA_FLAG_SIGNED_NES): the
+// Broad GSEA normalisation, each sign divided by the mean magnitude of the
+// null scores (permutations 1..B-1, NaN skipped) of that sign.  NP is the
+// reference's two-sided count either way.
+template <class T, bool SIGNED>
 __global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
                             T *__restrict__ out, T *__restrict__ factor,
                             const int32_t *__restrict__ budget,
@@ -137,39 +144,60 @@ __global__ void k_set_stats(const T *__restrict__ es, int64_t P, int G,
     const int64_t B = budget ? (int64_t)budget[s] : P;
     const T e0 = es[s];
     const T ref = __builtin_fabs(e0);
-    T sum = T(0);
+    T sum = T(0), sneg = T(0);
+    int64_t npos = 0, nneg = 0;
     double s1 = 0.0, s2 = 0.0;
     int64_t c = 0;
     int64_t p = 0;
-    // 32 loads in flight per lane; the sum stays sequential in p
+    // 32 loads in flight per lane; the sums stay sequential in p
     constexpr int U = 32;
     for (; p + U <= B; p += U) {
         T v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = __builtin_fabs(es[(p + u) * G + s]);
+        for (int u = 0; u < U; ++u) v[u] = es[(p + u) * G + s];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            sum += v[u];
-            c += v[u] >= ref;
-            if (cv) { s1 += (double)v[u]; s2 += (double)v[u] * v[u]; }
+            const T a = __builtin_fabs(v[u]);
+            if (!SIGNED) {
+                sum += a;
+            } else if (p + u > 0) {
+                if (v[u] < T(0)) { sneg += a; ++nneg; }
+                else if (v[u] >= T(0)) { sum += a; ++npos; }
+            }
+            c += a >= ref;
+            if (cv) { s1 += (double)a; s2 += (double)a * a; }
         }
     }
     for (; p < B; ++p) {
-        const T v = __builtin_fabs(es[p * G + s]);
-        sum += v;
-        c += v >= ref;
-        if (cv) { s1 += (double)v; s2 += (double)v * v; }
+        const T raw = es[p * G + s];
+        const T a = __builtin_fabs(raw);
+        if (!SIGNED) {
+            sum += a;
+        } else if (p > 0) {
+            if (raw < T(0)) { sneg += a; ++nneg; }
+            else if (raw >= T(0)) { sum += a; ++npos; }
+        }
+        c += a >= ref;
+        if (cv) { s1 += (double)a; s2 += (double)a * a; }
     }
-    const T f = (T)B / sum;
+    const T fp = SIGNED ? (T)npos / sum : (T)B / sum;
+    const T fn = SIGNED ? (T)nneg / sneg : fp;
     out[s] = e0;
-    out[G + s] = e0 * f;
+    out[G + s] = e0 * (e0 < T(0) ? fn : fp);
     out[2 * G + s] = (T)c / (T)B;
-    factor[s] = f;
+    factor[s] = fp;
+    factor[G + s] = fn;
     if (cv) {
         const double m = s1 / (double)B;
         const double var = s2 / (double)B - m * m;
         cv[s] = (float)(m > 0.0 && var > 0.0 ? sqrt(var) / m : 0.0);
     }
+}
+
+// a score normalised by its set's factor of its sign (statistics.cuh:90)
+template <class T>
+__device__ __forceinline__ T normalised(T v, const T *__restrict__ factor, int G, int s) {
+    return v * factor[v < T(0) ? G + s : s];
 }
 
 // per permutation: extrema of normalised ES over sets (statistics.cuh:85-97),
@@ -185,7 +213,7 @@ __global__ void k_perm_extrema(const T *__restrict__ es, int64_t P, int G,
     T mx = -__builtin_inf(), mn = __builtin_inf();
     for (int s = lane; s < G; s += kWave) {
         if (budget && p >= budget[s]) continue;
-        const T v = es[p * G + s] * factor[s];
+        const T v = normalised(es[p * G + s], factor, G, s);
         mx = mx < v ? v : mx;
         mn = mn > v ? v : mn;
     }
@@ -434,7 +462,7 @@ __global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
                 if (p >= budget[s]) continue;
                 w = (uint32_t)(P / budget[s]);
             }
-            const T z = es[p * G + s] * factor[s];
+            const T z = normalised(es[p * G + s], factor, G, s);
             if (z != z) continue;
             int lb;
             if (LDS && nb > 0) {

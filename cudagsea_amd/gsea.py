@@ -83,11 +83,19 @@ def map_sets(gene_symbols, gene_sets):
     return out_off, genes[:total.value]
 
 
+def _flags(ascending, swap_labels, signed_nes):
+    return ((_lib.FLAG_ASCENDING if ascending else 0) |
+            (_lib.FLAG_SWAP_LABELS if swap_labels else 0) |
+            (_lib.FLAG_SIGNED_NES if signed_nes else 0))
+
+
 def run(X, labels, set_offsets, set_genes, num_perms, metric,
         ascending=False, swap_labels=False, double=False, want_es=False,
-        timings=False):
+        timings=False, signed_nes=False):
     """Whole path on host arrays.  X is sample-major [S][N].
-    Returns stats [5][G] (and es [G][P] if want_es, timings dict if asked)."""
+    Returns stats [5][G] (and es [G][P] if want_es, timings dict if asked).
+    signed_nes: Broad GSEA's per-sign normalisation of NES (GSEA_FLAG_SIGNED_NES)
+    instead of the reference's single factor."""
     dt = np.float64 if double else np.float32
     X = np.ascontiguousarray(X, dtype=dt)
     S, N = X.shape
@@ -100,8 +108,7 @@ def run(X, labels, set_offsets, set_genes, num_perms, metric,
     mid = metric_id(metric)
     stats = np.empty((5, G), dtype=dt)
     es = np.empty((G, num_perms), dtype=dt) if want_es else None
-    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
-        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    flags = _flags(ascending, swap_labels, signed_nes)
     t = _lib.gsea_timings()
     fn = lib().gsea_run_f64 if double else lib().gsea_run_f32
     check(fn(_ptr(X), N, S, _ptr(labels), _ptr(set_offsets), _ptr(set_genes),
@@ -123,15 +130,14 @@ def _dp(t):
 
 def run_device(X, labels, set_offsets, set_genes, num_perms, metric,
                stats_out, es_out=None, ascending=False, swap_labels=False,
-               stream=None):
+               stream=None, signed_nes=False):
     """All arrays are device tensors except `labels` (host, S bytes).
     X [S][N] float32/float64, set CSR int32, stats_out [5][G], es_out [G][P]."""
     S, N = X.shape
     G = set_offsets.numel() - 1
     double = X.dtype.itemsize == 8
     labels = np.ascontiguousarray(labels, dtype=np.uint8)
-    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
-        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    flags = _flags(ascending, swap_labels, signed_nes)
     t = _lib.gsea_timings()
     fn = lib().gsea_run_device_f64 if double else lib().gsea_run_device_f32
     st = ctypes.c_void_p(stream) if stream else None
@@ -142,7 +148,8 @@ def run_device(X, labels, set_offsets, set_genes, num_perms, metric,
 
 
 def sweep_device(X, labels, set_offsets, set_genes, num_perms, metrics, stats_out,
-                 ascending=False, swap_labels=False, graph=True, stream=None):
+                 ascending=False, swap_labels=False, graph=True, stream=None,
+                 signed_nes=False):
     """Several metrics over the same device inputs (gsea_sweep_device_*):
     stats_out is a [len(metrics)][5][G] device tensor.  With graph=True the
     launch sequence is one HIP graph, captured on the first call and replayed
@@ -152,8 +159,7 @@ def sweep_device(X, labels, set_offsets, set_genes, num_perms, metrics, stats_ou
     double = X.dtype.itemsize == 8
     labels = np.ascontiguousarray(labels, dtype=np.uint8)
     mids = np.ascontiguousarray([metric_id(m) for m in metrics], dtype=np.int32)
-    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
-        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    flags = _flags(ascending, swap_labels, signed_nes)
     t = _lib.gsea_timings()
     fn = lib().gsea_sweep_device_f64 if double else lib().gsea_sweep_device_f32
     st = ctypes.c_void_p(stream) if stream else None
@@ -165,7 +171,7 @@ def sweep_device(X, labels, set_offsets, set_genes, num_perms, metrics, stats_ou
 
 def run_budget_device(X, labels, set_offsets, set_genes, num_perms, metric,
                       budget, stats_out, es_pg_out=None, extension=None,
-                      ascending=False, swap_labels=False, stream=None):
+                      ascending=False, swap_labels=False, stream=None, signed_nes=False):
     """Budgeted run: set s is scored for permutations 0..budget[s]-1 (host
     int array, each entry dividing num_perms).  extension = (fdr, z,
     nes_tol) or None (see gsea_run_budget_device_f32).  Returns (final budgets, timings).  es_pg_out, if
@@ -182,8 +188,7 @@ def run_budget_device(X, labels, set_offsets, set_genes, num_perms, metric,
     ext = None
     if extension is not None:
         ext = _lib.gsea_extension(*[float(v) for v in extension])
-    flags = (_lib.FLAG_ASCENDING if ascending else 0) | (
-        _lib.FLAG_SWAP_LABELS if swap_labels else 0)
+    flags = _flags(ascending, swap_labels, signed_nes)
     t = _lib.gsea_timings()
     fn = lib().gsea_run_budget_device_f64 if double else lib().gsea_run_budget_device_f32
     st = ctypes.c_void_p(stream) if stream else None

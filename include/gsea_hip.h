@@ -46,6 +46,13 @@ extern "C" {
 /* flags */
 #define GSEA_FLAG_ASCENDING   1   /* reference sort_direction == false */
 #define GSEA_FLAG_SWAP_LABELS 2   /* reference swap_labels == true */
+/* Broad GSEA normalisation instead of statistics.cuh:59-70's single factor
+ * P / sum_p |ES_p|: a score >= 0 is divided by the mean of its set's null
+ * scores >= 0, a score < 0 by the mean magnitude of those < 0 (null =
+ * permutations 1..P-1, NaN skipped; an empty side gives NaN).  NES, FWER and
+ * FDR follow the normalised scores; ES and NP are unchanged.  No
+ * counterpart in the reference, which has the single factor only. */
+#define GSEA_FLAG_SIGNED_NES  4
 
 /* engine limits */
 #define GSEA_MAX_GENES   65535   /* 16-bit rank rows */

@@ -93,10 +93,18 @@ void orc_scores_ext_f64(const double *sorted, const uint32_t *index,
  * ES, NES, NP, FWER, FDR. */
 void orc_final_statistics_f32(float *es, int64_t G, int64_t P, float *out);
 void orc_final_statistics_f64(double *es, int64_t G, int64_t P, double *out);
+/* flags bit2: signed NES (Broad GSEA's per-sign normalisation by the mean of
+ * the null scores of each sign, permutations 1..P-1; defined here, the
+ * reference has the single factor only). */
+void orc_final_statistics_ext_f32(float *es, int64_t G, int64_t P, int flags,
+                                  float *out);
+void orc_final_statistics_ext_f64(double *es, int64_t G, int64_t P, int flags,
+                                  double *out);
 
 /* ---- whole path: cugsea.cuh:267-453 ----
  * flags bit0: ascending sort (reference sort_direction=false),
- *       bit1: swap phenotype labels (cudaGSEA_wrapper.cu:246-251).
+ *       bit1: swap phenotype labels (cudaGSEA_wrapper.cu:246-251),
+ *       bit2: signed NES (as orc_final_statistics_ext).
  * es_out[G][P] (may be NULL) receives the raw ES before normalisation.
  * Returns 0, or 1 for an unknown metric, or 21 for a bad argument. */
 int orc_gsea_f32(const float *X, int64_t N, int64_t S, const uint8_t *labels,

@@ -74,6 +74,8 @@ def lib():
                 _P, _P, _P, _P, _I64, _I64, _I64, _P, _P, _P]
             getattr(L, "orc_final_statistics_" + sfx).argtypes = [
                 _P, _I64, _I64, _P]
+            getattr(L, "orc_final_statistics_ext_" + sfx).argtypes = [
+                _P, _I64, _I64, ctypes.c_int, _P]
             f = getattr(L, "orc_gsea_" + sfx)
             f.argtypes = [_P, _I64, _I64, _P, _P, _P, _I64, _I64,
                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
@@ -191,7 +193,7 @@ def scores_ext(sorted_keys, index, offsets, genes):
 
 
 def gsea_ext(X, labels, offsets, genes, P, metric, ascending=False, swap=False,
-             double=False):
+             double=False, signed_nes=False):
     """Whole CPU path with each walk's extremes: (stats, es, hi, lo)."""
     dt = _dt(double)
     X = np.ascontiguousarray(X, dtype=dt)
@@ -201,23 +203,25 @@ def gsea_ext(X, labels, offsets, genes, P, metric, ascending=False, swap=False,
     r = correlate(X, lab, P, metric, sample_major=True, double=double)
     sk, idx = pathway_sort(r, ascending=ascending)
     es, hi, lo = scores_ext(sk, idx, offsets, genes)
-    st, _ = final_statistics(es.copy())
+    st, _ = final_statistics(es.copy(), signed_nes=signed_nes)
     return st, es, hi, lo
 
 
-def final_statistics(es):
-    """Returns (stats [5][G], normalised es)."""
+def final_statistics(es, signed_nes=False):
+    """Returns (stats [5][G], normalised es).  signed_nes: Broad GSEA's
+    per-sign normalisation (orc_final_statistics_ext)."""
     es = np.array(es, copy=True, order="C")
     double = es.dtype == np.float64
     G, P = es.shape
     out = np.empty((5, G), dtype=es.dtype)
     sfx = "f64" if double else "f32"
-    getattr(lib(), "orc_final_statistics_" + sfx)(_ptr(es), G, P, _ptr(out))
+    getattr(lib(), "orc_final_statistics_ext_" + sfx)(_ptr(es), G, P,
+                                                      4 if signed_nes else 0, _ptr(out))
     return out, es
 
 
 def gsea(X, labels, offsets, genes, P, metric, ascending=False, swap=False,
-         sample_major=True, double=False, want_es=True):
+         sample_major=True, double=False, want_es=True, signed_nes=False):
     """Whole reference CPU path.  Returns (stats [5][G], es [G][P] or None)."""
     dt = _dt(double)
     X = np.ascontiguousarray(X, dtype=dt)
@@ -232,7 +236,7 @@ def gsea(X, labels, offsets, genes, P, metric, ascending=False, swap=False,
     mid = METRICS.index(metric) if isinstance(metric, str) else int(metric)
     stats = np.empty((5, G), dtype=dt)
     es = np.empty((G, P), dtype=dt) if want_es else None
-    flags = (1 if ascending else 0) | (2 if swap else 0)
+    flags = (1 if ascending else 0) | (2 if swap else 0) | (4 if signed_nes else 0)
     rc = getattr(lib(), "orc_gsea_" + ("f64" if double else "f32"))(
         _ptr(X), N, S, _ptr(labels), _ptr(offsets), _ptr(genes), G, P, mid,
         flags, int(sample_major), _ptr(es) if want_es else None, _ptr(stats))

@@ -20,12 +20,13 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-def _per_metric(gpu, X, w, P, metrics, tdt):
+def _per_metric(gpu, X, w, P, metrics, tdt, signed=False):
     import torch
     out = []
     for m in metrics:
         st = torch.empty((5, w["G"]), dtype=tdt, device="cuda")
-        gpu.run_device(X, w["labels"], dev(w["set_offsets"]), dev(w["set_genes"]), P, m, st)
+        gpu.run_device(X, w["labels"], dev(w["set_offsets"]), dev(w["set_genes"]), P, m, st,
+                       signed_nes=signed)
         out.append(st.cpu().numpy())
     return np.stack(out)
 
@@ -86,6 +87,25 @@ def test_sweep_graph_follows_inputs_and_recaptures(gpu):
     gpu.sweep_device(X, w["labels"], off, genes, 96, METRICS4[:2], st4)
     want4 = _per_metric(gpu, X, w3, 96, METRICS4[:2], torch.float32)
     assert bits_equal(st4.cpu().numpy(), want4).all()
+
+
+def test_sweep_graph_signed_nes(gpu):
+    # the flag is part of the captured sequence: a signed sweep replays the
+    # signed statistics, and switching the flag back recaptures
+    import torch
+    w = workloads.make("c3", num_genes=1200, num_sets=90, num_perms=160)
+    P, G = 160, w["G"]
+    X = dev(w["X"])
+    off, genes = dev(w["set_offsets"]), dev(w["set_genes"])
+    want_s = _per_metric(gpu, X, w, P, METRICS4[:2], torch.float32, signed=True)
+    want_r = _per_metric(gpu, X, w, P, METRICS4[:2], torch.float32)
+    st = torch.empty((2, 5, G), dtype=torch.float32, device="cuda")
+    for rep in range(2):
+        gpu.sweep_device(X, w["labels"], off, genes, P, METRICS4[:2], st, signed_nes=True)
+        assert bits_equal(st.cpu().numpy(), want_s).all(), rep
+    gpu.sweep_device(X, w["labels"], off, genes, P, METRICS4[:2], st)
+    assert bits_equal(st.cpu().numpy(), want_r).all()
+    assert not np.allclose(want_s[:, 1], want_r[:, 1])
 
 
 def test_statistics_host_threshold_fallback_matches_oracle(gpu):

@@ -248,6 +248,28 @@ def test_full_path_flags(gpu):
             parity.assert_stats(got, want, es_want, hi, lo)
 
 
+@pytest.mark.parametrize("dbl", [False, True])
+def test_signed_nes_full_path(gpu, dbl):
+    # GSEA_FLAG_SIGNED_NES: the GPU's ES against the oracle's walks, then the
+    # GPU's statistics against the oracle's signed statistics of those same
+    # ES, bit for bit (the same sums in the same order, the same products and
+    # counts, the FDR ratio in double)
+    print("signed NES", "fp64" if dbl else "fp32", flush=True)
+    w = small("c3", num_genes=1500, num_sets=150, num_perms=256)
+    P = 256
+    args = (w["X"], w["labels"], w["set_offsets"], w["set_genes"], P, "stable_signal2noise")
+    _, es_want, hi, lo = oracle.gsea_ext(*args, double=dbl, signed_nes=True)
+    got, es_got = gpu.run(*args, double=dbl, want_es=True, signed_nes=True)
+    parity.assert_es(es_got, es_want, hi, lo)
+    want, _ = oracle.final_statistics(es_got, signed_nes=True)
+    same = bits_equal(got, want)
+    assert same.all(), (np.argwhere(~same)[:5].tolist(), got[~same][:5], want[~same][:5])
+    # ES and NP are the reference's; NES follows the per-sign factors
+    ref = gpu.run(*args, double=dbl)
+    assert bits_equal(got[0], ref[0]).all() and bits_equal(got[2], ref[2]).all()
+    assert not np.allclose(got[1], ref[1])
+
+
 def test_gsea_r_mirror_and_dump(gpu, tmp_path):
     import pandas as pd
     w = small("c1", num_perms=64)

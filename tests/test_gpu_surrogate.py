@@ -175,18 +175,18 @@ def test_null_moments_match_numpy(gpu):
 
 
 # ------------------------------------------------------ budgeted run ----
-def _plain(w, P, metric):
+def _plain(w, P, metric, signed=False):
     import torch
     G = w["G"]
     stats = torch.empty((5, G), dtype=torch.float32, device="cuda")
     es = torch.empty((G, P), dtype=torch.float32, device="cuda")
     import cudagsea_amd as cg
     cg.run_device(dev(w["X"]), w["labels"], dev(w["set_offsets"]), dev(w["set_genes"]),
-                  P, metric, stats, es_out=es)
+                  P, metric, stats, es_out=es, signed_nes=signed)
     return stats.cpu().numpy(), es.cpu().numpy()
 
 
-def _budget(w, P, metric, budget, ext=None):
+def _budget(w, P, metric, budget, ext=None, signed=False):
     import torch
     import cudagsea_amd as cg
     G = w["G"]
@@ -194,7 +194,7 @@ def _budget(w, P, metric, budget, ext=None):
     es = torch.empty((P, G), dtype=torch.float32, device="cuda")
     bfin, _ = cg.run_budget_device(dev(w["X"]), w["labels"], dev(w["set_offsets"]),
                                    dev(w["set_genes"]), P, metric, budget, stats,
-                                   es_pg_out=es, extension=ext)
+                                   es_pg_out=es, extension=ext, signed_nes=signed)
     return stats.cpu().numpy(), es.cpu().numpy(), bfin
 
 
@@ -202,11 +202,13 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
 
 
-def test_budget_all_full_equals_plain(gpu):
+@pytest.mark.parametrize("signed", [False, True])
+def test_budget_all_full_equals_plain(gpu, signed):
     w = workloads.make("c3", num_genes=2500, num_samples=80, num_sets=120, seed=21)
     P = 512
-    sp, _ = _plain(w, P, "onepass_t_test")
-    sb, _, bfin = _budget(w, P, "onepass_t_test", np.full(w["G"], P, np.int32))
+    sp, _ = _plain(w, P, "onepass_t_test", signed)
+    sb, _, bfin = _budget(w, P, "onepass_t_test", np.full(w["G"], P, np.int32),
+                          signed=signed)
     assert np.array_equal(bits(sp), bits(sb))
     assert (bfin == P).all()
 

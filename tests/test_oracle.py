@@ -520,6 +520,87 @@ def test_statistics_invariants(orc):
     assert np.allclose(st[0], es[:, 0])
 
 
+# ------------------------------------------- signed NES (Broad GSEA) -------
+# The option divides a score >= 0 by the mean of its set's null scores >= 0
+# and a score < 0 by the mean magnitude of its null scores < 0 (Subramanian et
+# al., PNAS 2005, "normalized enrichment score"), null = permutations 1..P-1.
+@pytest.mark.parametrize("dbl", [False, True])
+def test_signed_nes_known_answers(orc, dbl):
+    dt = np.float64 if dbl else np.float32
+    es = np.array([[0.5, 1.0, 2.0, -1.0, -3.0],     # null + {1, 2}, - {1, 3}
+                   [-2.0, 1.0, 2.0, -1.0, -3.0],
+                   [0.0, 1.0, 2.0, -1.0, -3.0],
+                   [-1.0, 1.0, 2.0, 3.0, 0.0],      # no null score < 0
+                   [4.0, -8.0, 2.0, -4.0, -4.0]],   # the observed is not in the mean
+                  dtype=dt)
+    st, norm = orc.final_statistics(es, signed_nes=True)
+    nes = st[1].astype(np.float64)
+    assert nes[0] == pytest.approx(0.5 / 1.5, rel=1e-7)
+    assert nes[1] == -1.0
+    assert nes[2] == 0.0
+    assert np.isnan(nes[3])
+    assert nes[4] == 2.0
+    assert bits_equal(st[0], es[:, 0]).all()
+    # NP is the reference's two-sided count whatever the normalisation
+    ref, _ = orc.final_statistics(es)
+    assert bits_equal(st[2], ref[2]).all()
+    # the null scores of row 0 normalised per sign
+    assert np.allclose(norm[0], [1 / 3, 2 / 3, 4 / 3, -0.5, -1.5], rtol=1e-7)
+
+
+def test_signed_nes_null_means_are_one(orc):
+    # the defining property: per set and sign, the normalised null scores
+    # average to +1 / -1; the observed score (p = 0) is not in the mean
+    rng = np.random.default_rng(21)
+    es = rng.standard_normal((25, 300)) * 0.4 + rng.normal(0, 0.2, (25, 1))
+    es[:, 0] += 3.0
+    st, norm = orc.final_statistics(es, signed_nes=True)
+    null = norm[:, 1:]
+    for s in range(es.shape[0]):
+        v = null[s]
+        assert v[v >= 0].mean() == pytest.approx(1.0, rel=1e-12)
+        assert v[v < 0].mean() == pytest.approx(-1.0, rel=1e-12)
+    # and the observed row goes through the same factors
+    assert np.allclose(st[1], norm[:, 0], rtol=0, atol=0)
+
+
+def test_signed_nes_statistics_follow_the_normalised_scores(orc):
+    # FWER and FDR of the signed option: the same definitions over the
+    # per-sign normalised scores it returns (brute force in float64)
+    rng = np.random.default_rng(5)
+    es = (rng.standard_normal((40, 120)) * 0.3 + rng.normal(0, 0.1, (40, 1)))
+    es[:, 0] += np.linspace(-0.6, 0.6, 40)
+    st, norm = orc.final_statistics(es, signed_nes=True)
+    nes = st[1]
+    P = es.shape[1]
+    zmax, zmin = norm.max(axis=0), norm.min(axis=0)
+    fw = np.where(nes > 0, (zmax[None, :] >= nes[:, None]).sum(axis=1),
+                  (zmin[None, :] <= nes[:, None]).sum(axis=1)) / P
+    assert (st[3] == fw).all()
+    assert np.allclose(st[4], _fdr_brute(norm, nes), rtol=1e-12, equal_nan=True)
+    # the option changes the answers (a factor per sign, not one per set)
+    ref, _ = orc.final_statistics(es)
+    assert not np.allclose(ref[1], st[1])
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_statistics_sign_and_scale_symmetry(orc, signed):
+    # ES -> -ES swaps the two sides: NES changes sign, FWER / FDR / NP stay;
+    # ES -> 4 ES (exact) changes nothing but ES
+    rng = np.random.default_rng(8)
+    es = (rng.standard_normal((30, 90)) * 0.3).astype(np.float32)
+    es[:, 0] += np.linspace(-0.5, 0.5, 30).astype(np.float32)
+    assert (es != 0).all()
+    st, _ = orc.final_statistics(es, signed_nes=signed)
+    neg, _ = orc.final_statistics(-es, signed_nes=signed)
+    big, _ = orc.final_statistics(es * 4, signed_nes=signed)
+    assert bits_equal(neg[1], -st[1]).all()
+    for r in (2, 3, 4):
+        assert bits_equal(neg[r], st[r]).all(), r
+        assert bits_equal(big[r], st[r]).all(), r
+    assert bits_equal(big[1], st[1]).all()
+
+
 # --------------------------------------------------------- whole path ------
 def test_gsea_composes_stages(orc):
     from cudagsea_amd import workloads
