@@ -201,9 +201,11 @@ std::vector<proxy_strng_t> listCudaDevices_proxy() {
         char name[256] = {0};
         size_t bytes = 0;
         gsea_device_name(i, name, sizeof(name), &bytes);
-        // cudaGSEA_wrapper.cu:87-104 format
+        // cudaGSEA_wrapper.cu:87-104: "device <id>: <name> with <GiB> GiB RAM",
+        // the memory size rounded up to whole GiB
+        const size_t gib = (bytes + ((size_t)1 << 30) - 1) >> 30;
         out.push_back("device " + std::to_string(i) + ": " + name + " with " +
-                      std::to_string(bytes >> 30) + " GiB RAM");
+                      std::to_string(gib) + " GiB RAM");
     }
     return out;
 }
@@ -214,16 +216,17 @@ int getCudaDevice_proxy() {
     return d;
 }
 
+// cudaGSEA_wrapper.cu:126-137: an id that cannot be selected leaves the
+// current device active and prints the reference's fallback warning
 int setCudaDevice_proxy(int device) {
     int n = 0;
     gsea_device_count(&n);
-    if (device < 0 || device >= n) {
-        std::cout << "WARNING: device " << device
-                  << " not available, keeping the current device" << std::endl;
-        return getCudaDevice_proxy();
-    }
-    gsea_set_device(device);
-    return getCudaDevice_proxy();
+    if (device >= 0 && device < n) gsea_set_device(device);
+    const int active = getCudaDevice_proxy();
+    if (active != device)
+        std::cout << "WARNING: invalid device id, fallback to device " << active << "."
+                  << std::endl;
+    return active;
 }
 
 std::vector<float> gsea_single_proxy(std::vector<float> &exprs,

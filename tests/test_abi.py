@@ -41,6 +41,62 @@ def test_proxy_abi_symbols_exported():
         assert sym in out, sym
 
 
+CPP = os.path.join(ROOT, "tests", "cpp")
+PROXY_DEVICES = os.path.join(CPP, "_build", "proxy_devices_test")
+
+
+def test_proxy_device_functions_cpp_without_a_device():
+    # tests/cpp/proxy_devices_test.cpp links the proxy ABI from C++ and checks
+    # listCudaDevices_proxy / getCudaDevice_proxy / setCudaDevice_proxy
+    # against cudaGSEA_wrapper.cu:87-138; with no device visible the list is
+    # empty and every id but 0 falls back to device 0 with the warning
+    import subprocess
+    subprocess.run(["make", "-C", CPP], check=True, stdout=subprocess.PIPE)
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([PROXY_DEVICES], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, env=env, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "ok 0 device(s), 0 failure(s)"
+
+
+RPKG = os.path.join(ROOT, "cudagsea_amd", "rpkg", "cudaGSEA")
+R_EXPORTS = {"loadLabelsFromCLS", "loadGeneSetsFromGMT", "loadExpressionDataFromGCT",
+             "listCudaDevices", "getCudaDevice", "setCudaDevice", "GSEA"}
+
+
+def test_r_package_entry_points_are_consistent():
+    # R is not installed here: check the package's pieces against each other
+    # and against the reference package's exports (cudaGSEA/NAMESPACE): every
+    # export is an R function whose .Call target is an RcppExport of the glue
+    # taking as many SEXP arguments as the R call passes
+    ns = open(os.path.join(RPKG, "NAMESPACE")).read()
+    exports = set(re.findall(r"\w+", re.search(r"export\(([^)]*)\)", ns).group(1)))
+    assert exports == R_EXPORTS
+    assert re.search(r"useDynLib\(cudaGSEA\)", ns)
+    desc = open(os.path.join(RPKG, "DESCRIPTION")).read()
+    assert re.search(r"^Package: cudaGSEA$", desc, re.M)
+    assert re.search(r"^LinkingTo: Rcpp$", desc, re.M)
+    rsrc = open(os.path.join(RPKG, "R", "cudaGSEA.R")).read()
+    funcs = dict(re.findall(r"(\w+) <- function\(([^)]*)\)", rsrc))
+    assert set(funcs) == R_EXPORTS
+    calls = {m[0]: m[1] for m in re.findall(r'\.Call\("(\w+)"((?:,[^,)]*)*?),\s*PACKAGE = "cudaGSEA"\)',
+                                             rsrc.replace("\n", " "))}
+    assert set(calls) == R_EXPORTS
+    glue = open(os.path.join(RPKG, "src", "gsea_rcpp.cpp")).read()
+    for name, args in calls.items():
+        m = re.search(r"RcppExport SEXP %s\(([^)]*)\)" % name, glue)
+        assert m, name
+        n_c = len(re.findall(r"\bSEXP\b", m.group(1)))
+        n_r = len([a for a in args.split(",") if a.strip()])
+        assert n_c == n_r, (name, n_c, n_r)
+    # GSEA()'s arguments and defaults (cudaGSEA/R/GSEA.R)
+    assert re.sub(r"\s+", " ", funcs["GSEA"]) == (
+        "exprsData, labelList, geneSets, numPermutations, metricString = \"\", "
+        "dumpFileName = \"\", checkInput = TRUE, doublePrecision = FALSE")
+    mk = open(os.path.join(RPKG, "src", "Makevars")).read()
+    assert "-lgsea_hip" in mk and "GSEA_AMD_HOME" in mk
+
+
 def test_metric_names_roundtrip():
     import cudagsea_amd as cg
     L = cg.lib()

@@ -292,6 +292,31 @@ def test_gsea_r_mirror_and_dump(gpu, tmp_path):
 
 
 # ------------------------------------------------- genome-scale configs ----
+def test_proxy_device_functions_cpp(gpu):
+    # tests/cpp/proxy_devices_test (built with the library): the device
+    # proxies' strings, selection and fallback warning against
+    # cudaGSEA_wrapper.cu:87-138, checked in C++ against hipGetDeviceProperties;
+    # here also against torch's view of the same device
+    import os
+    import subprocess
+    import torch
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build",
+                       "proxy_devices_test")
+    assert os.path.exists(exe), "build the library first (it builds tests/cpp)"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    n = torch.cuda.device_count()
+    assert lines[-1] == "ok %d device(s), 0 failure(s)" % n
+    prop = torch.cuda.get_device_properties(0)
+    gib = (prop.total_memory + (1 << 30) - 1) >> 30
+    # the runtime's device name, or its ISA name where the runtime reports an
+    # empty name (torch then substitutes a generic one)
+    names = {prop.name, getattr(prop, "gcnArchName", prop.name)}
+    assert lines[0] in {"device 0: %s with %d GiB RAM" % (n, gib) for n in names}, lines[0]
+
+
 def test_c2_fp32_full_size_properties(gpu):
     import torch
     print("c2 fp32 full size", flush=True)
