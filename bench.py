@@ -9,8 +9,9 @@ steps / time.
 
 A second leg times the surrogate-guided run against the plain full run on
 c5 (identical FDR < 0.25 calls and max |dNES| < 0.02 are checked and
-reported).  Prints one JSON line with roofline, cpu_baseline and surrogate
-objects.
+reported), a third the whole path at the other configurations' full sizes
+(c1, c2 in fp32 and fp64, c4).  Prints one JSON line with roofline,
+cpu_baseline, surrogate and other_configs objects.
 """
 import argparse
 import json
@@ -181,6 +182,43 @@ def surrogate_leg(args):
     return out
 
 
+def configs_leg():
+    """Whole-path GPU rates of the other configurations at their full sizes
+    (c1; c2 in fp32 and fp64; c4 with its three metrics), inputs resident,
+    one graph-captured sweep per configuration timed after a warm-up, default
+    launch shapes."""
+    import torch
+    import cudagsea_amd as cg
+    from cudagsea_amd import workloads
+    cg.set_tuning()
+    out = []
+    for cfg, dbl in (("c1", False), ("c2", False), ("c2", True), ("c4", False)):
+        dt = np.float64 if dbl else np.float32
+        w = workloads.make(cfg, dtype=dt)
+        X = torch.from_numpy(w["X"]).cuda()
+        off = torch.from_numpy(w["set_offsets"]).cuda()
+        genes = torch.from_numpy(w["set_genes"]).cuda()
+        st = torch.empty((len(w["metrics"]), 5, w["G"]), dtype=X.dtype, device="cuda")
+
+        def run():
+            return cg.sweep_device(X, w["labels"], off, genes, w["P"], w["metrics"], st)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tm = run()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        n = w["G"] * w["P"] * len(w["metrics"])
+        out.append({"config": cfg, "dtype": "f64" if dbl else "f32",
+                    "metrics": w["metrics"], "genes": w["N"], "samples": w["S"],
+                    "sets": w["G"], "permutations": w["P"],
+                    "ms": round(el * 1e3, 3), "es_per_s": round(n / el, 1),
+                    "ms_es": round(tm["ms_es"], 3), "ms_metric": round(tm["ms_metric"], 3)})
+        print("config %s %s: %.1f ms" % (cfg, out[-1]["dtype"], el * 1e3),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +237,8 @@ def main():
                     help="search the launch shapes again (and cache them)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate-guided vs plain comparison (c5)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the GPU rates of the other configurations")
     ap.add_argument("--cpu-sizes", action="store_true",
                     help="only time the CPU oracle at the configurations' sizes")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
@@ -347,6 +387,10 @@ def main():
     if not args.no_surrogate:
         sur = surrogate_leg(args)
 
+    others = None
+    if not args.no_configs:
+        others = configs_leg()
+
     line = {
         "metric": "ES/sec (gene-set\u00d7perm) at 20k genes, 1 GPU; surrogate vs "
                   "plain at equal FDR",
@@ -361,6 +405,7 @@ def main():
                    "permutations": P, "metrics": len(metrics),
                    "mean_ms_per_metric_run": stage_ms},
         "roofline": roof, "tuning": tuning, "cpu_baseline": cpu, "surrogate": sur,
+        "other_configs": others,
     }
     print(json.dumps(line), flush=True)
 
