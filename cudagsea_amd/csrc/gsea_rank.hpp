@@ -417,20 +417,16 @@ __device__ __forceinline__ void es_mark_set(const uint16_t *sg, const uint16_t *
     if (lane == 0) atomicAnd(&bm[N >> 5], ~(1u << (N & 31)));
 }
 
-// One lane's walk over its hits in rank order.  The lane's W bitmap words
-// hold ranks [lane*W*32, (lane+1)*W*32); nz flags its nonempty words and j is
-// the set index of its first hit.  Every word read is cleared.  EXACT == false: L accumulates the lane's
-// sum of c from 0, vmin / vmax keep the smallest value just before a hit and
-// the largest just after one, of L + misses*decay (NaN means none yet:
-// fmin / fmax skip it, as the reference's comparison skips a NaN running
-// sum).  EXACT == true: L starts at the lane's Sa and every candidate goes
-// through es_candidates (first strict maximum, rounded to T).
-template <class T, bool EXACT, class AF>
-__device__ __forceinline__ void es_walk(uint32_t *bm, int W, AF &A,
-                                        const HitShare<T> &share, int lane,
-                                        uint32_t nz, int hits, int j, double dec,
-                                        double &L, double &vmin, double &vmax,
-                                        T &best) {
+// One lane's walk over its hits in rank order, first-strict-maximum rule
+// (the tie case of es_large_set).  The lane's W bitmap words hold ranks
+// [lane*W*32, (lane+1)*W*32); nz flags its nonempty words and j is the set
+// index of its first hit.  S starts at the lane's Sa (sum of c over the
+// hits of earlier lanes); every candidate goes through es_candidates.
+template <class T, class AF>
+__device__ __forceinline__ void es_walk_exact(uint32_t *bm, int W, AF &A,
+                                              const HitShare<T> &share, int lane,
+                                              uint32_t nz, int hits, int j, double dec,
+                                              double &S, T &best) {
     uint32_t cur = 0;
     int wq = 0;        // first rank of the current word
     int jt = j;        // set index of the current hit
@@ -444,22 +440,49 @@ __device__ __forceinline__ void es_walk(uint32_t *bm, int W, AF &A,
         }
         const int q = wq + __builtin_ctz(cur);
         cur &= cur - 1u;
-        const double c = share(A(q));
-        if (EXACT) {
-            es_candidates<T>(L, c, q, jt, dec, best);
-        } else {
-            // running sum just before the hit, then just after it
-            const double before = __builtin_fma((double)(q - jt), dec, L);
-            vmin = min_skip_nan(vmin, before);
-            L += c;
-            vmax = max_skip_nan(vmax, before + c);
-        }
+        es_candidates<T>(S, share(A(q)), q, jt, dec, best);
     }
 }
 
+// The same walk for the lane's extremes.  With L the lane's running sum of c
+// and jt the set index of the hit, the running sum just before the hit at
+// rank q is L + (q - jt)*decay and just after it that plus c.  The walk
+// carries Y = L - jt*decay, so each hit costs one product with its rank:
+// before = q*decay + Y, after = c + before, Y += c - decay.  vmin keeps the
+// smallest value before a hit and vmax the largest after one (NaN means none
+// yet: v_min / v_max skip it, as the reference's comparison skips a NaN
+// running sum).  Every word read is cleared.  Returns L, the lane's sum of c.
+template <class T, class AF>
+__device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, double rn,
+                                                   int lane, uint32_t nz, int hits, int j,
+                                                   double dec, double &vmin, double &vmax) {
+    double Y = -((double)j * dec);
+    uint32_t cur = 0;
+    int wq = 0;
+    const int base = lane * W;
+    for (int t = 0; t < hits; ++t) {
+        if (cur == 0u) {
+            const int w = base + __builtin_ctz(nz);
+            nz &= nz - 1u;
+            cur = bm[w];
+            bm[w] = 0u;   // leaves the bitmap clear for the next set
+            wq = w * 32;
+        }
+        const int q = wq + __builtin_ctz(cur);
+        cur &= cur - 1u;
+        const double x = (double)A(q);
+        const double before = __builtin_fma((double)q, dec, Y);
+        vmin = min_skip_nan(vmin, before);
+        vmax = max_skip_nan(vmax, __builtin_fma(x, rn, before));
+        Y = __builtin_fma(x, rn, Y) - dec;
+    }
+    // L = Y + (j + hits)*decay; exactly 0 for a lane without hits
+    return hits ? Y + (double)(j + hits) * dec : 0.0;
+}
+
 // Larger sets: one wavefront per set.  Phase A sets the k rank bits in the
-// wavefront's N-bit bitmap (lane l owns words l*W .. l*W+W-1, so lanes own
-// consecutive rank ranges) and sums N_R.  Each lane then walks its own hits
+// wavefront's (N+1)-bit bitmap (lane l owns words l*W .. l*W+W-1, so lanes
+// own consecutive rank ranges) and sums N_R.  Each lane then walks its own hits
 // once, keeping its sum of c and the extremes of its local running sum; one
 // wavefront scan of the sums places every lane's extremes on the set's
 // running sum.  Rounding to T is monotone, so the largest (smallest) rounded
@@ -488,12 +511,13 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
     for (int i = 0; i < W; ++i) {
         const uint32_t w = bm[lane * W + i];
         hits += __popc(w);
-        nz |= (w != 0u ? 1u : 0u) << i;
+        nz |= min(w, 1u) << i;
     }
     const int j = wave_excl_scan(hits);
-    double L = 0.0, vmin = __builtin_nan(""), vmax = __builtin_nan("");
+    double vmin = __builtin_nan(""), vmax = __builtin_nan("");
     T best = T(0);
-    es_walk<T, false>(bm, W, A, share, lane, nz, hits, j, dec, L, vmin, vmax, best);
+    const double L = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, hits, j, dec, vmin,
+                                         vmax);
     const double Sa = wave_excl_scan(L);
     const T hi = wave_reduce_idem((T)(Sa + vmax), [](T x, T y) { return max_skip_nan(x, y); });
     const T lo = wave_reduce_idem((T)(Sa + vmin), [](T x, T y) { return min_skip_nan(x, y); });
@@ -506,7 +530,7 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
         es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, unused);
         wave_sync_lds();
         double S2 = Sa;
-        es_walk<T, true>(bm, W, A, share, lane, nz, hits, j, dec, S2, vmin, vmax, best);
+        es_walk_exact<T>(bm, W, A, share, lane, nz, hits, j, dec, S2, best);
         best = wave_first_absmax(best);
     } else {
         best = ah > al ? hi : (al > ah ? lo : T(0));
