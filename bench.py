@@ -239,6 +239,9 @@ def main():
                     help="skip the surrogate-guided vs plain comparison (c5)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the GPU rates of the other configurations")
+    ap.add_argument("--no-default-step", action="store_true",
+                    help="skip the untimed default-shape step (profiling runs: every "
+                         "dispatch then uses the tuned shapes)")
     ap.add_argument("--cpu-sizes", action="store_true",
                     help="only time the CPU oracle at the configurations' sizes")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
@@ -311,9 +314,11 @@ def main():
         return time.perf_counter() - t0, tims
 
     # the default shapes, for the tuned-vs-default comparison (untimed steps)
-    cg.set_tuning()
-    step()
-    default_s, _ = timed(1)
+    default_s = None
+    if not args.no_default_step:
+        cg.set_tuning()
+        step()
+        default_s, _ = timed(1)
     cg.set_tuning(**knobs)
     for i in range(args.warmup):
         step()
@@ -374,7 +379,7 @@ def main():
                 "es": round(ms_es / n_runs, 3),
                 "stats": round(ms_stats / n_runs, 3)}
     tuning = {"knobs": knobs, "source": source,
-              "default_ms_per_step": round(default_s * 1e3, 2),
+              "default_ms_per_step": round(default_s * 1e3, 2) if default_s else None,
               "tuned_ms_per_step": round(elapsed * 1e3 / args.steps, 2),
               "graph": "one HIP graph per step (%d metrics), replayed" % len(metrics)}
 
