@@ -837,13 +837,17 @@ static int enqueue_stats(const T *es_pg, int64_t P, int64_t G, T *d_stats,
         HIP_TRY(hipGetLastError());
     }
     if (P > 1) {
-        const int blocks = (int)std::min<int64_t>(P - 1, 4LL * d->cu_count);
         if (b.fdr_lds) {
-            hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(256), b.fdr_lds, st,
+            // the thresholds, bucket table and two histograms fill most of a
+            // CU's LDS, so one 16-wavefront workgroup per CU hides the ES
+            // row loads' latency
+            const int blocks = (int)std::min<int64_t>(P - 1, d->cu_count);
+            hipLaunchKernelGGL((k_fdr_hist<T, true>), dim3(blocks), dim3(1024), b.fdr_lds, st,
                                es_pg, P, g, (const T *)b.factor, d_budget, (const T *)b.thr,
                                (const FdrInfo<T> *)b.info, (const int32_t *)b.table, hub, hlb,
                                signs);
         } else {
+            const int blocks = (int)std::min<int64_t>(P - 1, 4LL * d->cu_count);
             hipLaunchKernelGGL((k_fdr_hist<T, false>), dim3(blocks), dim3(256), 0, st,
                                es_pg, P, g, (const T *)b.factor, d_budget, (const T *)b.thr,
                                (const FdrInfo<T> *)b.info, (const int32_t *)b.table, hub, hlb,

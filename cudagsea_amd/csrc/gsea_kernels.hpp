@@ -424,7 +424,7 @@ __global__ void k_fwer_final(T *__restrict__ out, int64_t P, int G, const T *__r
 // LDS=true keeps thresholds and per-workgroup histograms in LDS; LDS=false
 // searches and counts in global memory (large numbers of sets).
 template <class T, bool LDS>
-__global__ void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
+__global__ __launch_bounds__(1024) void k_fdr_hist(const T *__restrict__ es, int64_t P, int G,
                            const T *__restrict__ factor,
                            const int32_t *__restrict__ budget,
                            const T *__restrict__ thr, const FdrInfo<T> *__restrict__ info,
