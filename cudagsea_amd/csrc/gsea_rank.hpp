@@ -444,19 +444,23 @@ __device__ __forceinline__ void es_walk_exact(uint32_t *bm, int W, AF &A,
     }
 }
 
-// The same walk for the lane's extremes.  With L the lane's running sum of c
-// and jt the set index of the hit, the running sum just before the hit at
-// rank q is L + (q - jt)*decay and just after it that plus c.  The walk
-// carries Y = L - jt*decay, so each hit costs one product with its rank:
-// before = q*decay + Y, after = c + before, Y += c - decay.  vmin keeps the
-// smallest value before a hit and vmax the largest after one (NaN means none
-// yet: v_min / v_max skip it, as the reference's comparison skips a NaN
-// running sum).  Every word read is cleared.  Returns L, the lane's sum of c.
+// The same walk for the lane's extremes, relative to the lane's start.
+// With L the lane's running sum of c and t the number of its earlier hits,
+// the set's running sum just before the lane's hit at rank q is
+// Sa + L + (q - j - t)*decay (Sa: sum of c over earlier lanes, j: their hit
+// count) and just after it that plus c.  The walk carries Y = L - t*decay,
+// so each hit costs one product with its rank: before = q*decay + Y,
+// after = c + before, Y += c - decay; the lane's offset Sa - j*decay is added
+// afterwards.  vmin keeps the smallest value before a hit and vmax the
+// largest after one (NaN means none yet: v_min / v_max skip it, as the
+// reference's comparison skips a NaN running sum).  Every word read is
+// cleared.  Returns the final Y, L - hits*decay: the exclusive wavefront
+// scan of it is each lane's offset Sa - j*decay.
 template <class T, class AF>
 __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, double rn,
-                                                   int lane, uint32_t nz, int hits, int j,
+                                                   int lane, uint32_t nz, int hits,
                                                    double dec, double &vmin, double &vmax) {
-    double Y = -((double)j * dec);
+    double Y = 0.0;
     uint32_t cur = 0;
     int wq = 0;
     const int base = lane * W;
@@ -476,8 +480,7 @@ __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, d
         vmax = max_skip_nan(vmax, __builtin_fma(x, rn, before));
         Y = __builtin_fma(x, rn, Y) - dec;
     }
-    // L = Y + (j + hits)*decay; exactly 0 for a lane without hits
-    return hits ? Y + (double)(j + hits) * dec : 0.0;
+    return Y;
 }
 
 // Larger sets: one wavefront per set.  Phase A sets the k rank bits in the
@@ -513,14 +516,12 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
         hits += __popc(w);
         nz |= min(w, 1u) << i;
     }
-    const int j = wave_excl_scan(hits);
     double vmin = __builtin_nan(""), vmax = __builtin_nan("");
     T best = T(0);
-    const double L = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, hits, j, dec, vmin,
-                                         vmax);
-    const double Sa = wave_excl_scan(L);
-    const T hi = wave_reduce_idem((T)(Sa + vmax), [](T x, T y) { return max_skip_nan(x, y); });
-    const T lo = wave_reduce_idem((T)(Sa + vmin), [](T x, T y) { return min_skip_nan(x, y); });
+    const double Y = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, hits, dec, vmin, vmax);
+    const double off = wave_excl_scan(Y);   // Sa - j*decay of each lane
+    const T hi = wave_reduce_idem((T)(off + vmax), [](T x, T y) { return max_skip_nan(x, y); });
+    const T lo = wave_reduce_idem((T)(off + vmin), [](T x, T y) { return min_skip_nan(x, y); });
     const T ah = hi == hi ? fabs(hi) : T(0);
     const T al = lo == lo ? fabs(lo) : T(0);
     if (ah == al && ah != T(0)) {
@@ -529,7 +530,8 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
         double unused = 0.0;
         es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, unused);
         wave_sync_lds();
-        double S2 = Sa;
+        const int j = wave_excl_scan(hits);
+        double S2 = off + (double)j * dec;   // Sa
         es_walk_exact<T>(bm, W, A, share, lane, nz, hits, j, dec, S2, best);
         best = wave_first_absmax(best);
     } else {
