@@ -368,7 +368,12 @@ template <class T, int B, class AF>
 __device__ __forceinline__ void es_mark_hits(__amdgpu_buffer_rsrc_t rs, const uint16_t *pos,
                                              AF &A, uint32_t *bm, int j0, int lane,
                                              double &nr) {
-    const int vo = 2 * (j0 + lane);   // byte offset; slot u adds 128u
+    // byte offset; slot u adds 128u, folded into the loads' immediate offset
+    // (an opaque base keeps the compiler from rebasing the slots on the last
+    // one, which needs a vector add per slot for the negative offsets).  The
+    // range check of the buffer covers voffset + immediate offset.
+    int vo = 2 * (j0 + lane);
+    asm volatile("" : "+v"(vo));
     uint32_t q[B];
 #pragma unroll
     for (int u = 0; u < B; ++u)
