@@ -6,15 +6,20 @@ The reference fixes these choices: 256-thread metric blocks
 (enrichment_scores.cuh:240-245) and a free-memory batch size
 (batch_scheduler.cuh:27-33).  Here they are measured on the workload itself:
 
-  metric_gene_tile x metric_perms_per_block   timed by the metric stage
+  metric_gene_tile x metric_perms_per_block   per accumulator kind, timed by
+                                              the metric stage of one metric
+                                              of that kind
   es_lane_set_max x es_waves                  timed by the enrichment stage
 
-Each group is searched exhaustively with the other group at its best (two
+Each group is searched exhaustively with the others at their best (two
 rounds of coordinate descent) on a permutation subset of the real inputs
 (stage times are linear in the permutation count), timing the stages with the
-engine's own HIP events, best of `reps` runs.  The sort's radix width is not
-a knob: k_rank's 8-bit digits keep the 16 wavefronts' digit counters within
-LDS beside the keys, and the stage is 4 % of a run.
+engine's own HIP events, best of `reps` runs.  The accumulator kinds differ
+in where their time goes (the naive sums are bound by the LDS reads of X and
+prefer wide tiles streamed over many permutations, the Welford and
+compensated kinds by arithmetic), hence one tile per kind.  The sort's radix
+width is not a knob: k_rank's 8-bit digits keep the 16 wavefronts' digit
+counters within LDS beside the keys.
 
 Results are cached in tuning_cache.json next to this file, keyed by genes,
 samples, sets, permutations, precision, metric kind mix and the device's CU
@@ -36,7 +41,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE_PATH = os.path.join(HERE, "tuning_cache.json")
 
 METRIC_SPACE = {"metric_gene_tile": [16, 32, 64],
-                "metric_perms_per_block": [64, 128, 256, 512]}
+                "metric_perms_per_block": [128, 256, 512, 1024, 2048]}
 ES_SPACE = {"es_lane_set_max": [-1, 32, 64, 128],
             "es_waves": [16, 14, 12]}
 
@@ -101,7 +106,17 @@ def _stage_ms(X, labels, off, genes, P, metrics, knobs, stat, reps):
     return total
 
 
-def tune(X, labels, off, genes, num_perms, metrics, sample_perms=2048, reps=2,
+def _with_kind(knobs, kind, gt, ppb):
+    """knobs with kind's metric tile set (per-kind lists of six values)."""
+    out = dict(knobs)
+    tiles = list(out.get("metric_gene_tile_kind", [0] * 6))
+    ppbs = list(out.get("metric_perms_per_block_kind", [0] * 6))
+    tiles[kind], ppbs[kind] = int(gt), int(ppb)
+    out["metric_gene_tile_kind"], out["metric_perms_per_block_kind"] = tiles, ppbs
+    return out
+
+
+def tune(X, labels, off, genes, num_perms, metrics, sample_perms=4096, reps=2,
          log=None):
     """Search the knobs on the first min(num_perms, sample_perms)
     permutations; returns {"knobs", "default_ms", "tuned_ms", "trials",
@@ -110,27 +125,43 @@ def tune(X, labels, off, genes, num_perms, metrics, sample_perms=2048, reps=2,
     P = int(min(num_perms, sample_perms))
     scale = num_perms / P
     say = log or (lambda *a: None)
-    # the enrichment stage hardly depends on the metric: tune it on one
+    # one metric per accumulator kind times that kind's tile; the enrichment
+    # stage hardly depends on the metric: tune it on one
+    by_kind = {}
+    for m in metrics:
+        by_kind.setdefault(_acc_kind(m), m)
     es_metrics = metrics[:1]
     best = {}
     trials = 0
     t0 = time.perf_counter()
     for rnd in range(2):
-        for space, stat, ms in ((METRIC_SPACE, ("ms_metric",), metrics),
-                                (ES_SPACE, ("ms_es",), es_metrics)):
-            keys = list(space)
-            cands = [dict(zip(keys, v)) for v in np.array(np.meshgrid(
-                *[space[k] for k in keys])).T.reshape(-1, len(keys)).tolist()]
+        for kind, m in sorted(by_kind.items()):
             scored = []
-            for c in cands:
-                knobs = dict(best)
-                knobs.update({k: int(v) for k, v in c.items()})
-                scored.append((_stage_ms(X, labels, off, genes, P, ms, knobs, stat, reps),
-                               knobs))
-                trials += 1
+            for gt in METRIC_SPACE["metric_gene_tile"]:
+                for ppb in METRIC_SPACE["metric_perms_per_block"]:
+                    knobs = _with_kind(best, kind, gt, ppb)
+                    scored.append((_stage_ms(X, labels, off, genes, P, [m], knobs,
+                                             ("ms_metric",), reps), knobs))
+                    trials += 1
             scored.sort(key=lambda x: x[0])
             best = scored[0][1]
-            say("round %d %s: best %s %.3f ms" % (rnd, stat[0], best, scored[0][0]))
+            say("round %d metric kind %d (%s): tile %d x %d, %.3f ms" % (
+                rnd, kind, m, best["metric_gene_tile_kind"][kind],
+                best["metric_perms_per_block_kind"][kind], scored[0][0]))
+        keys = list(ES_SPACE)
+        cands = [dict(zip(keys, v)) for v in np.array(np.meshgrid(
+            *[ES_SPACE[k] for k in keys])).T.reshape(-1, len(keys)).tolist()]
+        scored = []
+        for c in cands:
+            knobs = dict(best)
+            knobs.update({k: int(v) for k, v in c.items()})
+            scored.append((_stage_ms(X, labels, off, genes, P, es_metrics, knobs, ("ms_es",),
+                                     reps), knobs))
+            trials += 1
+        scored.sort(key=lambda x: x[0])
+        best = scored[0][1]
+        say("round %d ms_es: best %s %.3f ms" % (rnd, {k: best[k] for k in keys},
+                                                  scored[0][0]))
     stages = ("ms_metric", "ms_rank", "ms_es", "ms_stats")
     default_ms = _stage_ms(X, labels, off, genes, P, metrics, {}, stages, reps) * scale
     tuned_ms = _stage_ms(X, labels, off, genes, P, metrics, best, stages, reps) * scale
@@ -141,7 +172,7 @@ def tune(X, labels, off, genes, num_perms, metrics, sample_perms=2048, reps=2,
             "search_s": round(time.perf_counter() - t0, 1)}
 
 
-def tune_workload(w, double=False, sample_perms=2048, log=None, path=CACHE_PATH):
+def tune_workload(w, double=False, sample_perms=4096, log=None, path=CACHE_PATH):
     """Tune a workloads.make() workload and store the cache entry."""
     import torch
     dt = np.float64 if double else np.float32
@@ -159,7 +190,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--config", default="c3")
     ap.add_argument("--double", action="store_true")
-    ap.add_argument("--sample-perms", type=int, default=2048)
+    ap.add_argument("--sample-perms", type=int, default=4096)
     ap.add_argument("--cache", default=CACHE_PATH, help="cache file to update")
     args = ap.parse_args()
     from . import workloads

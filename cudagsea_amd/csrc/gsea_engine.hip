@@ -282,8 +282,9 @@ static int prepare_labels(const uint8_t *labels, int64_t S, int flags,
 // widest of 64/32/16 genes whose tile fits 64 KiB (two workgroups per CU);
 // otherwise X is read from global memory.
 template <class T>
-static int metric_tile(int64_t S, bool *xlds, int lds_max) {
-    const int tuned = g_tune.metric_gene_tile;
+static int metric_tile(int64_t S, int kind, bool *xlds, int lds_max) {
+    const int tuned = g_tune.metric_gene_tile_kind[kind] ? g_tune.metric_gene_tile_kind[kind]
+                                                         : g_tune.metric_gene_tile;
     if ((tuned == 16 || tuned == 32 || tuned == 64) &&
         (size_t)S * tuned * sizeof(T) <= (size_t)lds_max) {
         *xlds = true;
@@ -308,16 +309,20 @@ static int launch_metric_rcp(const T *dX, int64_t N, int64_t S, const Labels &L,
     a.Pb = (int)Pb; a.metric = metric; a.r = r;
     a.rcp = rcp;
     bool xlds = false;
-    const int gt = metric_tile<T>(S, &xlds, d->lds_max);
+    const int kind = acc_kind(metric);
+    const int gt = metric_tile<T>(S, kind, &xlds, d->lds_max);
     const int64_t gx = (N + gt - 1) / gt;
     // enough workgroups to fill the device several times over, each streaming
     // as many permutations as possible over its X tile
     const int64_t target = 8LL * d->cu_count;
     int64_t ppb = (Pb * gx + target - 1) / target;
     ppb = std::max<int64_t>(32, std::min<int64_t>(512, (ppb + 31) / 32 * 32));
-    if (g_tune.metric_perms_per_block > 0) ppb = g_tune.metric_perms_per_block;
+    if (g_tune.metric_perms_per_block_kind[kind] > 0)
+        ppb = g_tune.metric_perms_per_block_kind[kind];
+    else if (g_tune.metric_perms_per_block > 0)
+        ppb = g_tune.metric_perms_per_block;
     a.ppb = (int)std::min<int64_t>(ppb, std::max<int64_t>(Pb, 1));
-    HIP_TRY(launch_metric_kernel<T>(a, acc_kind(metric), gt, xlds, st));
+    HIP_TRY(launch_metric_kernel<T>(a, kind, gt, xlds, st));
     return GSEA_OK;
 }
 
@@ -1681,10 +1686,15 @@ int gsea_set_tuning(const gsea_tuning *t) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (t) {
         const int gt = t->metric_gene_tile, lm = t->es_lane_set_max;
-        if (t->batch_perms < 0 || t->metric_perms_per_block < 0 || t->es_waves < 0 ||
-            t->es_waves > kRankWaves || (gt && gt != 16 && gt != 32 && gt != 64) ||
-            (lm > 0 && lm != 8 && lm != 16 && lm != 32 && lm != 64 && lm != 128) || lm < -1)
-            return fail(GSEA_ERR_INVALID_ARGUMENT, "tuning value out of range");
+        bool bad = t->batch_perms < 0 || t->metric_perms_per_block < 0 || t->es_waves < 0 ||
+                   t->es_waves > kRankWaves || (gt && gt != 16 && gt != 32 && gt != 64) ||
+                   (lm > 0 && lm != 8 && lm != 16 && lm != 32 && lm != 64 && lm != 128) ||
+                   lm < -1;
+        for (int k = 0; k < 6; ++k) {
+            const int g = t->metric_gene_tile_kind[k];
+            bad |= (g && g != 16 && g != 32 && g != 64) || t->metric_perms_per_block_kind[k] < 0;
+        }
+        if (bad) return fail(GSEA_ERR_INVALID_ARGUMENT, "tuning value out of range");
     }
     g_tune = t ? *t : gsea_tuning{};
     return GSEA_OK;

@@ -44,10 +44,18 @@ def _cstrings(strs):
 
 
 def set_tuning(**kw):
-    """Launch-shape tuning (gsea_set_tuning); no arguments = automatic."""
+    """Launch-shape tuning (gsea_set_tuning); no arguments = automatic.
+    metric_gene_tile_kind / metric_perms_per_block_kind take six values, one
+    per accumulator kind (0: the kind-independent knob)."""
     t = _lib.gsea_tuning()
     for k, v in kw.items():
-        setattr(t, k, int(v))
+        if k in _lib.gsea_tuning.KIND_KNOBS:
+            vals = [int(x) for x in v]
+            if len(vals) != 6:
+                raise ValueError("%s takes 6 values, one per accumulator kind" % k)
+            setattr(t, k, (ctypes.c_int16 * 6)(*vals))
+        else:
+            setattr(t, k, int(v))
     check(lib().gsea_set_tuning(ctypes.byref(t)))
 
 

@@ -152,6 +152,32 @@ def test_argument_errors_before_any_device_call():
     assert e.value.code == 31            # above GSEA_MAX_GENES
 
 
+def test_tuning_roundtrip_and_range_checks():
+    # gsea_set_tuning / gsea_get_tuning need no device: the per-kind metric
+    # tiles round-trip, and values outside the documented ranges are refused
+    import cudagsea_amd as cg
+    from cudagsea_amd import _lib
+    assert ctypes.sizeof(_lib.gsea_tuning) == 72
+    try:
+        cg.set_tuning(metric_gene_tile=32, metric_gene_tile_kind=[64, 0, 16, 0, 32, 0],
+                      metric_perms_per_block_kind=[1024, 0, 0, 256, 0, 0], es_waves=12)
+        t = cg.get_tuning()
+        assert t["metric_gene_tile"] == 32 and t["es_waves"] == 12
+        assert t["metric_gene_tile_kind"] == [64, 0, 16, 0, 32, 0]
+        assert t["metric_perms_per_block_kind"] == [1024, 0, 0, 256, 0, 0]
+        for bad in ({"metric_gene_tile_kind": [0, 0, 24, 0, 0, 0]},
+                    {"metric_perms_per_block_kind": [0, -1, 0, 0, 0, 0]},
+                    {"metric_gene_tile": 48}, {"es_waves": 17}, {"es_lane_set_max": 100}):
+            with pytest.raises(cg.GseaError) as e:
+                cg.set_tuning(**bad)
+            assert e.value.code == 30, bad
+        with pytest.raises(ValueError):
+            cg.set_tuning(metric_gene_tile_kind=[16, 16])
+    finally:
+        cg.set_tuning()
+    assert cg.get_tuning()["metric_gene_tile_kind"] == [0] * 6
+
+
 def _frame():
     import pandas as pd
     rng = np.random.default_rng(0)

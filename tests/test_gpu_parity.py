@@ -58,6 +58,31 @@ def test_metric_bit_exact_all_metrics(gpu, config, N, S, dbl):
         assert eq.all(), (m, int((~eq).sum()), np.argwhere(~eq)[:3])
 
 
+def test_metric_tiles_per_kind_do_not_change_results(gpu):
+    # every launch shape computes the same arithmetic: per-kind tiles and
+    # permutation blocks give the default shapes' bits
+    import torch
+    w = small("c3", num_genes=3000, num_samples=120, num_perms=300)
+    X = dev(w["X"])
+    P, N = 300, w["N"]
+    metrics = ["naive_log2_ratio_of_classes", "stable_t_test", "onepass_signal2noise",
+               "twopass_t_test", "stable_signal2noise", "overkill_t_test"]
+    want = {}
+    for m in metrics:
+        r = torch.empty((P, N), dtype=torch.float32, device="cuda")
+        gpu.metric_device(X, w["labels"], P, m, r)
+        want[m] = r.cpu().numpy()
+    try:
+        gpu.set_tuning(metric_gene_tile_kind=[64, 32, 16, 64, 32, 16],
+                       metric_perms_per_block_kind=[1024, 96, 256, 32, 512, 2048])
+        for m in metrics:
+            r = torch.empty((P, N), dtype=torch.float32, device="cuda")
+            gpu.metric_device(X, w["labels"], P, m, r)
+            assert bits_equal(r.cpu().numpy(), want[m]).all(), m
+    finally:
+        gpu.set_tuning()
+
+
 def test_metric_log_bitwise_over_wide_range(gpu):
     # drive the log2-ratio metric through ratios spanning many binades,
     # subnormal and special results included
