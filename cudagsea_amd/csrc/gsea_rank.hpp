@@ -439,8 +439,8 @@ __device__ __forceinline__ void es_walk_exact(uint32_t *bm, int W, AF &A,
         if (cur == 0u) {
             const int w = lane * W + __builtin_ctz(nz);
             nz &= nz - 1u;
-            cur = bm[w];
-            bm[w] = 0u;   // leaves the bitmap clear for the next set
+            // one LDS swap reads the word and leaves it clear for the next set
+            cur = atomicExch(&bm[w], 0u);
             wq = w * 32;
         }
         const int q = wq + __builtin_ctz(cur);
@@ -473,8 +473,8 @@ __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, d
         if (cur == 0u) {
             const int w = base + __builtin_ctz(nz);
             nz &= nz - 1u;
-            cur = bm[w];
-            bm[w] = 0u;   // leaves the bitmap clear for the next set
+            // one LDS swap reads the word and leaves it clear for the next set
+            cur = atomicExch(&bm[w], 0u);
             wq = w * 32;
         }
         const int q = wq + __builtin_ctz(cur);
