@@ -459,17 +459,20 @@ __device__ __forceinline__ void es_walk_exact(uint32_t *bm, int W, AF &A,
 // afterwards.  vmin keeps the smallest value before a hit and vmax the
 // largest after one (NaN means none yet: v_min / v_max skip it, as the
 // reference's comparison skips a NaN running sum).  Every word read is
-// cleared.  Returns the final Y, L - hits*decay: the exclusive wavefront
-// scan of it is each lane's offset Sa - j*decay.
+// cleared.  Returns the final Y, L - hits*decay (hits: the lane's hit count):
+// the exclusive wavefront scan of it is each lane's offset Sa - j*decay.
 template <class T, class AF>
 __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, double rn,
-                                                   int lane, uint32_t nz, int hits,
+                                                   int lane, uint32_t nz,
                                                    double dec, double &vmin, double &vmax) {
     double Y = 0.0;
     uint32_t cur = 0;
     int wq = 0;
     const int base = lane * W;
-    for (int t = 0; t < hits; ++t) {
+    if (nz == 0u) return Y;
+    // one hit per step; the lane is done when its current word and nz are
+    // both empty (one 64-bit compare of the pair)
+    do {
         if (cur == 0u) {
             const int w = base + __builtin_ctz(nz);
             nz &= nz - 1u;
@@ -484,7 +487,7 @@ __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, d
         vmin = min_skip_nan(vmin, before);
         vmax = max_skip_nan(vmax, __builtin_fma(x, rn, before));
         Y = __builtin_fma(x, rn, Y) - dec;
-    }
+    } while ((((uint64_t)nz << 32) | cur) != 0u);
     return Y;
 }
 
@@ -512,18 +515,14 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
     const double dec = k == a.N ? 0.0 : decay;
 
     wave_sync_lds();
-    int hits = 0;
+    // the lane's nonempty words (W <= 32)
     uint32_t nz = 0;
     const int W = a.W;
 #pragma unroll 4
-    for (int i = 0; i < W; ++i) {
-        const uint32_t w = bm[lane * W + i];
-        hits += __popc(w);
-        nz |= min(w, 1u) << i;
-    }
+    for (int i = 0; i < W; ++i) nz |= min(bm[lane * W + i], 1u) << i;
     double vmin = __builtin_nan(""), vmax = __builtin_nan("");
     T best = T(0);
-    const double Y = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, hits, dec, vmin, vmax);
+    const double Y = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, dec, vmin, vmax);
     const double off = wave_excl_scan(Y);   // Sa - j*decay of each lane
     const T hi = wave_reduce_idem((T)(off + vmax), [](T x, T y) { return max_skip_nan(x, y); });
     const T lo = wave_reduce_idem((T)(off + vmin), [](T x, T y) { return min_skip_nan(x, y); });
@@ -535,6 +534,8 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
         double unused = 0.0;
         es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, unused);
         wave_sync_lds();
+        int hits = 0;
+        for (int i = 0; i < W; ++i) hits += __popc(bm[lane * W + i]);
         const int j = wave_excl_scan(hits);
         double S2 = off + (double)j * dec;   // Sa
         es_walk_exact<T>(bm, W, A, share, lane, nz, hits, j, dec, S2, best);
