@@ -40,12 +40,12 @@ class gsea_extension(ctypes.Structure):
 class gsea_tuning(ctypes.Structure):
     _fields_ = [("batch_perms", ctypes.c_int64), ("metric_gene_tile", ctypes.c_int32),
                 ("metric_perms_per_block", ctypes.c_int32), ("es_waves", ctypes.c_int32),
-                ("es_lane_set_max", ctypes.c_int32), ("reserved32", ctypes.c_int32),
+                ("es_lane_set_max", ctypes.c_int32), ("es_pair_set_max", ctypes.c_int32),
                 ("metric_gene_tile_kind", ctypes.c_int16 * 6),
                 ("metric_perms_per_block_kind", ctypes.c_int16 * 6),
                 ("reserved", ctypes.c_int64 * 2)]
     KNOBS = ("batch_perms", "metric_gene_tile", "metric_perms_per_block", "es_waves",
-             "es_lane_set_max")
+             "es_lane_set_max", "es_pair_set_max")
     KIND_KNOBS = ("metric_gene_tile_kind", "metric_perms_per_block_kind")
 
     def as_dict(self):

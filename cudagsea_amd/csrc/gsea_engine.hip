@@ -80,7 +80,7 @@ enum TierBuf { kTierIds, kTierGm, kTierDecay, kTierGenes, kNumTierBufs };
 struct TierKey {
     const int32_t *off = nullptr, *genes = nullptr;
     int64_t G = -1, N = -1;
-    int lane_max = -1;
+    int lane_max = -1, pair_max = -1;
     size_t tsize = 0;
     unsigned long long sum = 0, total = 0;
 };
@@ -407,12 +407,14 @@ static int plan_rank(int64_t N, const DeviceState *d, RankPlan *pl) {
     return fail(GSEA_ERR_UNSUPPORTED_SIZE, "gene count does not fit the ES kernel");
 }
 
-// Work split of the ES phase: sets above kSmallSet genes are scored by one
-// wavefront each (in set order), the others by one lane each in chunks of
-// 64 sets of one size class (<= 8, <= 16, <= 32, <= 64, <= 128 genes).
+// Work split of the ES phase: sets above the pair class are scored by one
+// wavefront each (in set order), sets of at most kSmallSet genes by one lane
+// each in chunks of 64 sets of one size class (<= 8, <= 16, <= 32, <= 64,
+// <= 128 genes), and sets of 129..2*kSmallSet genes by a pair of lanes each
+// in chunks of 32 sets (lane 2i holds entries 0..127, lane 2i+1 the rest).
 struct SetTiers {
     int n_large = 0, n_chunks = 0, chunk_end8 = 0, chunk_end16 = 0, chunk_end32 = 0;
-    int chunk_end64 = 0;
+    int chunk_end64 = 0, chunk_end128 = 0;
     int stride = 0;
     const int4 *large_desc = nullptr;
     const int32_t *small_ids = nullptr, *small_k = nullptr, *small_kmax = nullptr;
@@ -438,15 +440,20 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
         for (int64_t s = 0; s < G; ++s) all[(size_t)s] = (int32_t)s;
         subset = &all;
     }
-    // sets up to lane_max genes: one lane each (tuned: 8..128, or none)
+    // sets up to lane_max genes: one lane each (tuned: 8..128, or none);
+    // up to pair_max: a pair of lanes each (with lane_max 128 only)
     const int lane_max = g_tune.es_lane_set_max < 0 ? 0
                        : g_tune.es_lane_set_max > 0 ? std::min(g_tune.es_lane_set_max, kSmallSet)
                        : kSmallSet;
-    std::vector<int32_t> large, cls[5];
+    const int pair_max = lane_max != kSmallSet || g_tune.es_pair_set_max < 0 ? 0
+                       : g_tune.es_pair_set_max > 0 ? g_tune.es_pair_set_max
+                       : 2 * kSmallSet;
+    std::vector<int32_t> large, cls[6];
     for (int32_t s : *subset) {
         const int k = off[(size_t)s + 1] - off[(size_t)s];
         if (k < 0) return fail(GSEA_ERR_INVALID_ARGUMENT, "set_offsets must be non-decreasing");
-        if (k > lane_max) large.push_back((int32_t)s);
+        if (k > lane_max && k <= pair_max) cls[5].push_back((int32_t)s);
+        else if (k > lane_max) large.push_back((int32_t)s);
         else cls[k <= 8 ? 0 : k <= 16 ? 1 : k <= 32 ? 2 : k <= 64 ? 3 : 4].push_back((int32_t)s);
     }
     // large sets stay in set order: a workgroup's wavefronts claim items in
@@ -459,14 +466,21 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
             return off[x + 1] - off[x] < off[y + 1] - off[y];
         });
     std::vector<int32_t> sid, sk;
-    int ends[5];
-    for (int c = 0; c < 5; ++c) {
-        for (int32_t s : cls[c]) { sid.push_back(s); sk.push_back(off[s + 1] - off[s]); }
+    int ends[6];
+    for (int c = 0; c < 6; ++c) {
+        for (int32_t s : cls[c]) {
+            // a pair set takes two adjacent slots, both with its id and size
+            for (int h = 0; h < (c == 5 ? 2 : 1); ++h) {
+                sid.push_back(s);
+                sk.push_back(off[s + 1] - off[s]);
+            }
+        }
         while (sid.size() % kWave) { sid.push_back(-1); sk.push_back(0); }
         ends[c] = (int)(sid.size() / kWave);
     }
     t->n_large = (int)large.size();
-    t->n_chunks = ends[4];
+    t->n_chunks = ends[5];
+    t->chunk_end128 = ends[4];
     t->chunk_end8 = ends[0];
     t->chunk_end16 = ends[1];
     t->chunk_end32 = ends[2];
@@ -531,7 +545,8 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     }
     if (ns) {
         hipLaunchKernelGGL(k_small_gm, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
-                           d_off, d_genes, t->small_ids, (int)ns, (uint16_t *)pg);
+                           d_off, d_genes, t->small_ids, (int)ns, ends[4] * kWave,
+                           (uint16_t *)pg);
         HIP_TRY(hipGetLastError());
     }
     // the host vectors above must outlive the asynchronous copy
@@ -562,12 +577,14 @@ static int cached_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     TierKey k;
     k.off = d_off; k.genes = d_genes; k.G = G; k.N = N;
     k.lane_max = g_tune.es_lane_set_max;
+    k.pair_max = g_tune.es_pair_set_max;
     k.tsize = sizeof(T);
     k.sum = sum[0];
     k.total = sum[1];
     const TierKey &c = d->tier_key;
     if (d->tier_valid && c.off == k.off && c.genes == k.genes && c.G == k.G && c.N == k.N &&
-        c.lane_max == k.lane_max && c.tsize == k.tsize && c.sum == k.sum && c.total == k.total) {
+        c.lane_max == k.lane_max && c.pair_max == k.pair_max && c.tsize == k.tsize &&
+        c.sum == k.sum && c.total == k.total) {
         *out = g_tier0[dev];
         return GSEA_OK;
     }
@@ -608,6 +625,7 @@ static void fill_rank_args(RankArgs<T> &a, const RankPlan &pl, int64_t N, int64_
     a.chunk_end16 = tiers ? tiers->chunk_end16 : 0;
     a.chunk_end32 = tiers ? tiers->chunk_end32 : 0;
     a.chunk_end64 = tiers ? tiers->chunk_end64 : 0;
+    a.chunk_end128 = tiers ? tiers->chunk_end128 : 0;
     a.decay = tiers ? (const T *)tiers->decay : nullptr;
 }
 
@@ -1084,6 +1102,7 @@ struct SweepKey {
                metrics == o.metrics && flags == o.flags && stats == o.stats && st == o.st &&
                tsize == o.tsize && std::memcmp(&tune, &o.tune, sizeof(tune)) == 0 &&
                a.sum == b.sum && a.total == b.total && a.lane_max == b.lane_max &&
+               a.pair_max == b.pair_max &&
                generation == o.generation;
     }
 };
@@ -1686,10 +1705,11 @@ int gsea_set_tuning(const gsea_tuning *t) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (t) {
         const int gt = t->metric_gene_tile, lm = t->es_lane_set_max;
+        const int pm = t->es_pair_set_max;
         bool bad = t->batch_perms < 0 || t->metric_perms_per_block < 0 || t->es_waves < 0 ||
                    t->es_waves > kRankWaves || (gt && gt != 16 && gt != 32 && gt != 64) ||
                    (lm > 0 && lm != 8 && lm != 16 && lm != 32 && lm != 64 && lm != 128) ||
-                   lm < -1;
+                   lm < -1 || pm < -1 || (pm > 0 && (pm <= kSmallSet || pm > 2 * kSmallSet));
         for (int k = 0; k < 6; ++k) {
             const int g = t->metric_gene_tile_kind[k];
             bad |= (g && g != 16 && g != 32 && g != 64) || t->metric_perms_per_block_kind[k] < 0;

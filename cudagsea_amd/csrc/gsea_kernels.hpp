@@ -70,15 +70,18 @@ __global__ void k_large_genes(const int32_t *__restrict__ off,
 }
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
-// read row t of 64 consecutive slots; rows t >= k hold the sentinel
+// read row t of 64 consecutive slots; rows t >= k hold the sentinel.  Slots
+// from pair0 on hold pair sets: an odd slot has the set's entries from
+// kSmallSet on.
 __global__ void k_small_gm(const int32_t *__restrict__ off,
                            const int32_t *__restrict__ genes,
-                           const int32_t *__restrict__ small_ids, int nslots,
+                           const int32_t *__restrict__ small_ids, int nslots, int pair0,
                            uint16_t *__restrict__ gm) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= nslots) return;
     const int s = small_ids[slot];
-    const int o0 = s >= 0 ? off[s] : 0;
+    const int skip = slot >= pair0 && (slot & 1) ? kSmallSet : 0;
+    const int o0 = s >= 0 ? off[s] + skip : 0;
     const int k = s >= 0 ? off[s + 1] - o0 : 0;
     for (int t = 0; t < kSmallSet; ++t)
         gm[(size_t)t * nslots + slot] = (uint16_t)(t < k ? genes[o0 + t] + 1 : 0);

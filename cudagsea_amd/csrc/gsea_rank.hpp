@@ -64,6 +64,7 @@ struct RankArgs {
     const int32_t *small_kmax;         // [n_chunks] largest set of each chunk
     int small_stride;
     int chunk_end8, chunk_end16, chunk_end32, chunk_end64;   // chunk class boundaries
+    int chunk_end128;                  // chunks from here on: pair sets (129..256 genes)
     const T *decay;                    // [G] (T)(-1.0 / (double)(N - k_s))
 };
 
@@ -360,6 +361,147 @@ __device__ __forceinline__ void es_small_sets(const RankArgs<T> &a,
     if (sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
 }
 
+// Sets of 129..256 genes: a pair of lanes each (lanes 2i and 2i+1 of a
+// chunk of 32 sets).  Lane 2i gathers the set's entries 0..127, lane 2i+1
+// the rest (rows past the set's end hold the sentinel, rank N), and each
+// sorts its 128 ranks in registers; lane 2i+1 sorts descending (ascending on
+// complemented keys).  The two sequences then form one bitonic sequence of
+// 256 ranks, and one compare-exchange of key i of the even lane with key i of
+// the odd lane (a DPP move, a packed min and max) leaves the 128 smallest
+// ranks on the even lane and the 128 largest on the odd lane, each as a
+// bitonic sequence that a size-128 bitonic merge sorts ascending.  Each lane
+// then walks its 128 ranks as the one-lane path does, with global hit
+// indices (the odd lane's start at 128) and its own running sum from 0; the
+// odd lane's extremes are offset by the even lane's final sum, and the pair
+// combines its extremes through one more DPP move each.  N_R is the sum of
+// the two lanes' partial sums (the same addition on both lanes).
+template <class T, class AF>
+__device__ __forceinline__ void es_pair_sets(const RankArgs<T> &a, const uint16_t *pos,
+                                             AF &A, int64_t gp, int chunk, int lane) {
+    constexpr int NC = kSmallSet, R = NC / 2;
+    const int slot = chunk * kWave + lane;
+    const int sid = a.small_ids[slot];
+    const int k = a.small_k[slot];   // the whole set's size, on both lanes
+    const bool odd = (lane & 1) != 0;
+    const int t_base = odd ? NC : 0;  // set index of the lane's first entry
+    uint32_t v[R];
+    const uint32_t sent = (uint32_t)a.N * 0x00010001u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = sent;
+    const uint16_t *gcol = a.small_gm + slot;
+    int stride = a.small_stride;
+    asm volatile("" : "+s"(stride));
+    double nr = 0.0;
+#pragma unroll
+    for (int t0 = 0; t0 < NC; t0 += 8) {
+        int gg[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) gg[u] = gcol[(size_t)(t0 + u) * stride];
+        uint32_t g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = pos[gg[u]];
+        T x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = A((int)g[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u;
+            if (t < R) v[t] = (v[t] & 0xffff0000u) | g[u];
+            else v[t - R] = (v[t - R] & 0xffffu) | (g[u] << 16);
+            nr += (double)x[u];
+        }
+        asm volatile("" : "+v"(nr));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) asm volatile("" : "+v"(v[(t0 + u) % R]));
+    }
+    // the odd lane sorts its complemented keys, i.e. its ranks descending
+    const uint32_t flip = odd ? 0xffffffffu : 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] ^= flip;
+    bitonic_sort_u16x2<NC>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        v[r] ^= flip;
+        const uint32_t p = dpp_move_any<kDppQuadXor1, 0xf>(v[r]);
+        v[r] = odd ? pk_max_u16(v[r], p) : pk_min_u16(v[r], p);
+        asm volatile("" : "+v"(v[r]));
+    }
+    bitonic_step_u16x2<NC, NC, R>(v);
+    auto rank_at = [&](int t) -> int {
+        return t < R ? (int)(v[t] & 0xffffu) : (int)(v[t - R] >> 16);
+    };
+    nr = nr + dpp_move_any<kDppQuadXor1, 0xf>(nr);
+    const HitShare<T> share((T)nr);
+#pragma unroll
+    for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
+
+    const double decay = sid >= 0 ? (double)a.decay[sid] : 0.0;
+    const double dec = k == a.N ? 0.0 : decay;
+    const double qnan = __builtin_nan("");
+    double L = 0.0, vmin = qnan, vmax = qnan;
+#pragma unroll
+    for (int t0 = 0; t0 < NC; t0 += 8) {
+        T x[8];
+        int kk = k - t_base;   // the lane's entries
+        asm volatile("" : "+v"(kk));
+        int tb = t_base;
+        asm volatile("" : "+v"(tb));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = A(rank_at(t0 + u));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u;
+            const bool valid = t < kk;
+            const double before = __builtin_fma((double)(rank_at(t) - t - tb), dec, L);
+            vmin = min_skip_nan(vmin, valid ? before : qnan);
+            const double c = share(x[u]);   // 0 for a sentinel
+            L += c;
+            vmax = max_skip_nan(vmax, valid ? before + c : qnan);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // the even lane's sum of c offsets the odd lane's running sum
+    const double La = dpp_move_any<kDppQuadXor1, 0xf>(L);
+    const double off = odd ? La : 0.0;
+    vmin += off;
+    vmax += off;
+    vmin = min_skip_nan(vmin, dpp_move_any<kDppQuadXor1, 0xf>(vmin));
+    vmax = max_skip_nan(vmax, dpp_move_any<kDppQuadXor1, 0xf>(vmax));
+    const T hi = (T)vmax, lo = (T)vmin;
+    const T ah = hi == hi ? fabs(hi) : T(0);
+    const T al = lo == lo ? fabs(lo) : T(0);
+    T best = ah > al ? hi : (al > ah ? lo : T(0));
+    if (ah == al && ah != T(0)) {
+        // equal magnitudes: the first strictly largest candidate, in order,
+        // on each lane, then the even lane's unless the odd lane's is larger
+        T b = T(0);
+        double Sa = off;
+        // reload keys and |metric| here: reusing the walk's values would keep
+        // all 128 of each live across it
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int r = 0; r < R; ++r) asm volatile("" : "+v"(v[r]));
+#pragma unroll
+        for (int t0 = 0; t0 < NC; t0 += 8) {
+            int kk = k - t_base;
+            asm volatile("" : "+v"(kk));
+            int tb = t_base;
+            asm volatile("" : "+v"(tb));
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = t0 + u;
+                es_candidates<T>(Sa, share(A(rank_at(t))), rank_at(t), t + tb, decay, b,
+                                 t < kk);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const T bo = dpp_move_any<kDppQuadXor1, 0xf>(b);   // the partner's
+        const T be = odd ? bo : b, bd = odd ? b : bo;        // even's, odd's
+        best = fabs(bd) > fabs(be) ? bd : be;
+    }
+    if (!odd && sid >= 0) a.es[(size_t)gp * a.G + sid] = best;
+}
+
 // Phase A of a large set: B gene-id slots per lane (set entries j0 + 64u +
 // lane, u < B), the bits of their ranks set in the wavefront's bitmap and
 // their |r| summed.  rs covers the set's k entries: slots past them read 0,
@@ -491,6 +633,78 @@ __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, d
     return Y;
 }
 
+// The same walk for |metric| staged in LDS (fp32 runs), with the rank taken
+// from the LDS byte address of its |metric|, a_base + 4q: the address serves
+// both the gather and the product, as fma((double)(a_base + 4q), decay/4, Y).
+// decay/4 is exact, so for a_base = 0 each candidate has the bits of
+// fma((double)q, decay, Y); for a_base != 0 the candidates are offset by
+// a_base*decay/4, which the caller subtracts.  Each word's address base is
+// formed once per refill from the lane's (opaque) base.
+__device__ __forceinline__ double es_walk_extremes_lds(uint32_t *bm, int W, uint32_t a_base,
+                                                       double rn, int lane, uint32_t nz,
+                                                       double dec, double &vmin,
+                                                       double &vmax) {
+    double Y = 0.0;
+    uint32_t cur = 0;
+    uint32_t wq4 = 0;   // LDS byte address of |metric| at the word's first rank
+    uint32_t *bml = bm + lane * W;                        // the lane's words
+    uint32_t al = a_base + (uint32_t)(lane * W) * 128u;   // |metric| of its ranks
+    // opaque: one shift-add per refill instead of the compiler's re-derived
+    // (w + lane*W)*128 + a_base
+    asm volatile("" : "+v"(al));
+    const double dec4 = dec * 0.25;   // exact: a power-of-two scaling
+    if (nz == 0u) return Y;
+    do {
+        if (cur == 0u) {
+            const uint32_t w = (uint32_t)__builtin_ctz(nz);
+            nz &= nz - 1u;
+            // one LDS swap reads the word and leaves it clear for the next set
+            cur = atomicExch(&bml[w], 0u);
+            wq4 = al + w * 128u;
+        }
+        const uint32_t addr = wq4 + 4u * (uint32_t)__builtin_ctz(cur);
+        cur &= cur - 1u;
+        const double x =
+            (double)*(const __attribute__((address_space(3))) float *)(uintptr_t)addr;
+        const double before = __builtin_fma((double)addr, dec4, Y);
+        vmin = min_skip_nan(vmin, before);
+        vmax = max_skip_nan(vmax, __builtin_fma(x, rn, before));
+        Y = __builtin_fma(x, rn, Y) - dec;
+    } while ((((uint64_t)nz << 32) | cur) != 0u);
+    return Y;
+}
+
+// max and min over the 64 lanes at once, returned on every lane: the
+// butterfly of wave_sum with the combination folded into DPP-modified
+// v_max_f32 / v_min_f32 (IEEE mode: a NaN operand is skipped).  The two
+// chains alternate so that each DPP read of a register comes two
+// instructions after its write (the s_nop fills the second wait state).
+__device__ __forceinline__ void wave_maxmin_f32(float &hi, float &lo) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_max_f32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_max_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_max_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 row_half_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_max_f32_dpp %0, %0, %0 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 row_mirror row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_max_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_max_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "v_min_f32_dpp %1, %1, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "s_nop 1"
+        : "+v"(hi), "+v"(lo));
+    hi = read_lane(hi, kWave - 1);
+    lo = read_lane(lo, kWave - 1);
+}
+
 // Larger sets: one wavefront per set.  Phase A sets the k rank bits in the
 // wavefront's (N+1)-bit bitmap (lane l owns words l*W .. l*W+W-1, so lanes
 // own consecutive rank ranges) and sums N_R.  Each lane then walks its own hits
@@ -502,7 +716,7 @@ __device__ __forceinline__ double es_walk_extremes(uint32_t *bm, int W, AF &A, d
 // equal does the order of occurrence decide (the reference keeps the first):
 // that case walks the hits again with the first-strict-maximum rule.
 template <class T, bool ALDS, class AF>
-__device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
+__device__ __forceinline__ void es_large_set(const RankArgs<T> &a, const float *a_lds,
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
                                              int o0, int k, double decay, int lane) {
@@ -518,14 +732,41 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a,
     // the lane's nonempty words (W <= 32)
     uint32_t nz = 0;
     const int W = a.W;
+    // an opaque 1: min(word, 1) stays one v_min_u32 (with a literal 1 the
+    // compiler rewrites it as a compare and a select)
+    uint32_t one = 1u;
+    asm volatile("" : "+v"(one));
 #pragma unroll 4
-    for (int i = 0; i < W; ++i) nz |= min(bm[lane * W + i], 1u) << i;
+    for (int i = 0; i < W; ++i) nz |= min(bm[lane * W + i], one) << i;
     double vmin = __builtin_nan(""), vmax = __builtin_nan("");
     T best = T(0);
-    const double Y = es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, dec, vmin, vmax);
-    const double off = wave_excl_scan(Y);   // Sa - j*decay of each lane
-    const T hi = wave_reduce_idem((T)(off + vmax), [](T x, T y) { return max_skip_nan(x, y); });
-    const T lo = wave_reduce_idem((T)(off + vmin), [](T x, T y) { return min_skip_nan(x, y); });
+    double off;   // Sa - j*decay of each lane
+    if constexpr (ALDS) {
+        const uint32_t a_base =
+            (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) float *)a_lds;
+        off = wave_excl_scan(es_walk_extremes_lds(bm, W, a_base, share.rn, lane, nz, dec,
+                                                  vmin, vmax));
+        // the walk's frame is offset by a_base * decay / 4 (0 when the rows
+        // start at LDS byte 0)
+        const double o2 = a_base ? off - (double)a_base * 0.25 * dec : off;
+        vmin += o2;
+        vmax += o2;
+    } else {
+        off = wave_excl_scan(es_walk_extremes<T>(bm, W, A, share.rn, lane, nz, dec, vmin,
+                                                 vmax));
+        vmin += off;
+        vmax += off;
+    }
+    T hi, lo;
+    if constexpr (sizeof(T) == 4) {
+        float h = (float)vmax, l = (float)vmin;
+        wave_maxmin_f32(h, l);
+        hi = h;
+        lo = l;
+    } else {
+        hi = wave_reduce_idem((T)vmax, [](T x, T y) { return max_skip_nan(x, y); });
+        lo = wave_reduce_idem((T)vmin, [](T x, T y) { return min_skip_nan(x, y); });
+    }
     const T ah = hi == hi ? fabs(hi) : T(0);
     const T al = lo == lo ? fabs(lo) : T(0);
     if (ah == al && ah != T(0)) {
@@ -879,7 +1120,7 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                         (const __attribute__((address_space(4))) int *)(a.large_desc + item);
                     const T dec = *((const __attribute__((address_space(4))) T *)
                                         a.large_decay + item);
-                    es_large_set<T, ALDS>(a, pos, A, bm, gp, d[0], d[1], d[2], (double)dec,
+                    es_large_set<T, ALDS>(a, a_lds, pos, A, bm, gp, d[0], d[1], d[2], (double)dec,
                                           lane);
                 } else {
                     const int chunk = item - a.n_large;
@@ -891,8 +1132,10 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
                         es_small_sets<T, 32>(a, pos, A, gp, chunk, lane);
                     else if (chunk < a.chunk_end64)
                         es_small_sets<T, 64>(a, pos, A, gp, chunk, lane);
-                    else
+                    else if (chunk < a.chunk_end128)
                         es_small_sets<T, 128>(a, pos, A, gp, chunk, lane);
+                    else
+                        es_pair_sets<T>(a, pos, A, gp, chunk, lane);
                 }
                 item = __builtin_amdgcn_readfirstlane(next);
             }

@@ -89,19 +89,22 @@ typedef struct gsea_timings {
  *                      1..16 (default: as many as the LDS holds)
  *   es_lane_set_max    largest set scored by one lane (8, 16, 32, 64 or 128;
  *                      -1: every set by a wavefront; default 128)
+ *   es_pair_set_max    largest set scored by a pair of lanes (129..256; -1:
+ *                      none; default 256); with es_lane_set_max 128 only
  *   metric_gene_tile_kind[a], metric_perms_per_block_kind[a]
  *                      the two metric knobs for the metrics of accumulator
  *                      kind a (0 naive, 1 Kahan, 2 one-pass, 3 two-pass,
  *                      4 Welford, 5 compensated two-pass); 0: the value above
- * The per-kind fields occupy what earlier versions reserved (zero-filled),
- * so the structure's size and the other offsets are unchanged. */
+ * The per-kind fields and es_pair_set_max occupy what earlier versions
+ * reserved (zero-filled), so the structure's size and the other offsets are
+ * unchanged. */
 typedef struct gsea_tuning {
     int64_t batch_perms;
     int32_t metric_gene_tile;
     int32_t metric_perms_per_block;
     int32_t es_waves;
     int32_t es_lane_set_max;
-    int32_t reserved32;
+    int32_t es_pair_set_max;
     int16_t metric_gene_tile_kind[6];
     int16_t metric_perms_per_block_kind[6];
     int64_t reserved[2];

@@ -160,14 +160,18 @@ def test_tuning_roundtrip_and_range_checks():
     assert ctypes.sizeof(_lib.gsea_tuning) == 72
     try:
         cg.set_tuning(metric_gene_tile=32, metric_gene_tile_kind=[64, 0, 16, 0, 32, 0],
-                      metric_perms_per_block_kind=[1024, 0, 0, 256, 0, 0], es_waves=12)
+                      metric_perms_per_block_kind=[1024, 0, 0, 256, 0, 0], es_waves=12,
+                      es_pair_set_max=200)
         t = cg.get_tuning()
         assert t["metric_gene_tile"] == 32 and t["es_waves"] == 12
+        assert t["es_pair_set_max"] == 200
         assert t["metric_gene_tile_kind"] == [64, 0, 16, 0, 32, 0]
         assert t["metric_perms_per_block_kind"] == [1024, 0, 0, 256, 0, 0]
         for bad in ({"metric_gene_tile_kind": [0, 0, 24, 0, 0, 0]},
                     {"metric_perms_per_block_kind": [0, -1, 0, 0, 0, 0]},
-                    {"metric_gene_tile": 48}, {"es_waves": 17}, {"es_lane_set_max": 100}):
+                    {"metric_gene_tile": 48}, {"es_waves": 17}, {"es_lane_set_max": 100},
+                    {"es_pair_set_max": 128}, {"es_pair_set_max": 257},
+                    {"es_pair_set_max": -2}):
             with pytest.raises(cg.GseaError) as e:
                 cg.set_tuning(**bad)
             assert e.value.code == 30, bad

@@ -214,6 +214,75 @@ def test_rank_es_edge_sets(gpu):
         parity.assert_es(got, want, hi, lo)
 
 
+def _es_rows_case(gpu, r, sets, asc=False):
+    """GPU rank+ES stage on given metric rows r [P][N]; returns the GPU ES and
+    the oracle's (es, hi, lo), all [G][P]."""
+    import torch
+    off, genes = csr(sets)
+    sk, idx = oracle.pathway_sort(r, ascending=asc)
+    want, hi, lo = oracle.scores_ext(sk, idx, off, genes)
+    es = torch.empty((r.shape[0], len(sets)), dtype=torch.from_numpy(r).dtype,
+                     device="cuda")
+    gpu.rank_es_device(dev(r), dev(off), dev(genes), es, ascending=asc)
+    return es.cpu().numpy().T, want, hi, lo
+
+
+@pytest.mark.parametrize("dbl", [False, True])
+def test_rank_es_pair_sets(gpu, dbl):
+    # sets of 129..256 genes are scored by a pair of lanes (each sorts 128
+    # ranks, one cross-lane compare-exchange splits them): both ends of the
+    # class and its neighbours, sets holding the first and the last genes,
+    # both orders; then the same sets with the pair class switched off
+    print("pair sets", "fp64" if dbl else "fp32", flush=True)
+    rng = np.random.default_rng(11)
+    N, P = 3000, 24
+    dt = np.float64 if dbl else np.float32
+    r = rng.standard_normal((P, N)).astype(dt)
+    sets = [np.sort(rng.choice(N, k, replace=False)).astype(np.int32)
+            for k in (128, 129, 130, 160, 192, 200, 255, 256, 257)]
+    sets += [np.concatenate([np.arange(70), np.arange(N - 70, N)]).astype(np.int32),
+             np.arange(256, dtype=np.int32), np.arange(N - 129, N, dtype=np.int32)]
+    sets += [np.sort(rng.choice(N, int(k), replace=False)).astype(np.int32)
+             for k in rng.integers(129, 257, 40)]
+    try:
+        for pair_max in (0, 200, -1):
+            gpu.set_tuning(es_pair_set_max=pair_max)
+            for asc in (False, True):
+                got, want, hi, lo = _es_rows_case(gpu, r, sets, asc=asc)
+                parity.assert_es(got, want, hi, lo)
+    finally:
+        gpu.set_tuning()
+
+
+@pytest.mark.parametrize("dbl", [False, True])
+def test_rank_es_pair_sets_tie_rule(gpu, dbl):
+    # |largest| == |smallest| running sum: the reference keeps the first.
+    # Exact arithmetic: 256 genes whose |metric| is 2 (each hit adds
+    # 2/512 = 1/256) among N = 2,304 (each miss subtracts 1/2048); equal
+    # metrics rank by gene index, and gene g is given the g-th largest value.
+    # (a) the top 128 and the bottom 128 ranks: +1/2 after hit 127, -1/2
+    # just before hit 128, the odd lane's first; (b) the top 64 ranks, 1,024
+    # misses, 128 hits each followed by 8 misses, then the bottom 64 ranks:
+    # +1/4 after hit 63 and -1/4 just before hit 64, both on the even lane.
+    N, k, P = 2304, 256, 4
+    dt = np.float64 if dbl else np.float32
+    a = np.full(N, -2.0)
+    a[:128] = 2.0
+    a[128:N - 128] = np.linspace(1.9, -1.9, N - 256)
+    set_a = np.r_[0:128, N - 128:N]
+    b = np.full(N, -2.0)
+    b[:64] = 2.0
+    b[64:1088] = np.linspace(1.9, -1.9, 1024)
+    set_b = np.r_[0:64, 1088 + 9 * np.arange(128), N - 64:N]
+    for base, genes, top in ((a, set_a, 0.5), (b, set_b, 0.25)):
+        assert len(genes) == k and len(np.unique(genes)) == k
+        r = np.tile(base.astype(dt), (P, 1))
+        got, want, hi, lo = _es_rows_case(gpu, r, [genes.astype(np.int32)])
+        assert (hi == top).all() and (lo == -top).all(), "the case must tie exactly"
+        assert (want == top).all()
+        assert bits_equal(got, want).all(), (top, got.ravel(), want.ravel())
+
+
 @pytest.mark.parametrize("N", [2047, 2048, 4095, 4096, 32767, 32768])
 def test_rank_es_bitmap_capacity_boundaries(gpu, N):
     # the ES kernel's per-wavefront bitmap holds ranks 0..N (rank N is the
