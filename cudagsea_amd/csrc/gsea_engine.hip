@@ -458,8 +458,9 @@ static int build_tiers(const int32_t *d_off, const int32_t *d_genes, int64_t G,
     }
     // large sets stay in set order: a workgroup's wavefronts claim items in
     // order, so the ES stores of one permutation row fill cache lines
-    // together (with thousands of items per permutation the tail imbalance
-    // of this order is a few microseconds)
+    // together.  k_es claims the chunks first, longest class first, and the
+    // large sets last (each a few microseconds), so the wavefronts of a
+    // workgroup finish a permutation close together
     // within a class, sets of similar size share a chunk of 64 lanes
     for (auto &c : cls)
         std::stable_sort(c.begin(), c.end(), [&](int32_t x, int32_t y) {

@@ -52,8 +52,9 @@ struct RankArgs {
     int es_waves, nw;        // wavefronts that score sets, bitmap words (64*W)
     int W;                   // k_es: bitmap words per lane, (N + 1) bits in all
     size_t off_ids, off_cnt, off_misc;
-    // work items: n_large sets (> kSmallSet genes, one wavefront each), then
-    // chunks of 64 small sets (one lane each) grouped by size class
+    // work items: the chunks (64 lanes each: sets of one size class, one lane
+    // per set, or 32 pair sets), then the n_large sets above the pair class
+    // (one wavefront each)
     int n_items, n_large;
     const int4 *large_desc;            // [n_large] {set id, offset, size, 0},
                                        // in set order
@@ -1109,21 +1110,27 @@ __global__ __launch_bounds__(kRankThreads) void k_es(RankArgs<T> a) {
             int item = 0;
             if (lane == 0) item = atomicAdd(&misc[0], 1);
             item = __builtin_amdgcn_readfirstlane(item);
+            // items run longest first, so that the wavefronts' last items
+            // (which the workgroup waits for before the next permutation) are
+            // short: the chunks from the pair class down to the 8-gene class,
+            // then the large sets in set order
+            const int n_chunks = a.n_items - a.n_large;
             while (item < a.n_items) {
                 // fetch the next work item now; its latency hides behind this one
                 int next = 0;
                 if (lane == 0) next = atomicAdd(&misc[0], 1);
-                if (item < a.n_large) {
+                if (item >= n_chunks) {
+                    const int li = item - n_chunks;
                     // read-only tables through the constant address space:
                     // scalar loads of the uniform descriptor and decay
                     const __attribute__((address_space(4))) int *d =
-                        (const __attribute__((address_space(4))) int *)(a.large_desc + item);
+                        (const __attribute__((address_space(4))) int *)(a.large_desc + li);
                     const T dec = *((const __attribute__((address_space(4))) T *)
-                                        a.large_decay + item);
+                                        a.large_decay + li);
                     es_large_set<T, ALDS>(a, a_lds, pos, A, bm, gp, d[0], d[1], d[2], (double)dec,
                                           lane);
                 } else {
-                    const int chunk = item - a.n_large;
+                    const int chunk = n_chunks - 1 - item;
                     if (chunk < a.chunk_end8)
                         es_small_sets<T, 8>(a, pos, A, gp, chunk, lane);
                     else if (chunk < a.chunk_end16)
