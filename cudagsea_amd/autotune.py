@@ -9,7 +9,11 @@ The reference fixes these choices: 256-thread metric blocks
   metric_gene_tile x metric_perms_per_block   per accumulator kind, timed by
                                               the metric stage of one metric
                                               of that kind
-  es_lane_set_max x es_waves                  timed by the enrichment stage
+  es_lane_set_max x es_pair_set_max x es_waves
+                                              timed by the enrichment stage
+                                              (the pair class, sets of
+                                              129..256 genes on two lanes,
+                                              exists with lane split 128)
 
 Each group is searched exhaustively with the others at their best (two
 rounds of coordinate descent) on a permutation subset of the real inputs
@@ -43,7 +47,20 @@ CACHE_PATH = os.path.join(HERE, "tuning_cache.json")
 METRIC_SPACE = {"metric_gene_tile": [16, 32, 64],
                 "metric_perms_per_block": [128, 256, 512, 1024, 2048]}
 ES_SPACE = {"es_lane_set_max": [-1, 32, 64, 128],
+            "es_pair_set_max": [-1, 192, 256],
             "es_waves": [16, 14, 12]}
+
+
+def es_candidates():
+    """The enrichment-stage settings searched: every combination of
+    ES_SPACE, except pair-class settings with a lane split other than 128
+    (the pair class exists only with that split, so they would repeat the
+    pair-less run)."""
+    keys = list(ES_SPACE)
+    cands = [dict(zip(keys, v)) for v in np.array(np.meshgrid(
+        *[ES_SPACE[k] for k in keys])).T.reshape(-1, len(keys)).tolist()]
+    return [c for c in cands
+            if c["es_lane_set_max"] == 128 or c["es_pair_set_max"] == -1]
 
 
 def cu_count():
@@ -149,10 +166,8 @@ def tune(X, labels, off, genes, num_perms, metrics, sample_perms=4096, reps=2,
                 rnd, kind, m, best["metric_gene_tile_kind"][kind],
                 best["metric_perms_per_block_kind"][kind], scored[0][0]))
         keys = list(ES_SPACE)
-        cands = [dict(zip(keys, v)) for v in np.array(np.meshgrid(
-            *[ES_SPACE[k] for k in keys])).T.reshape(-1, len(keys)).tolist()]
         scored = []
-        for c in cands:
+        for c in es_candidates():
             knobs = dict(best)
             knobs.update({k: int(v) for k, v in c.items()})
             scored.append((_stage_ms(X, labels, off, genes, P, es_metrics, knobs, ("ms_es",),

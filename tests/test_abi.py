@@ -209,3 +209,21 @@ def test_gsea_unknown_metric_raises_instead_of_exit():
     with pytest.raises(cg.GseaError) as e:
         cg.GSEA(df, [0, 1, 0, 1], {"a": ["G1"]}, 10, "")
     assert e.value.code == 1
+
+
+def test_autotuner_es_search_space():
+    # every enrichment-stage candidate is a valid setting, each distinct run
+    # appears once: the pair class only with the lane split at 128
+    from cudagsea_amd import autotune
+    import cudagsea_amd as cg
+    cands = autotune.es_candidates()
+    keys = {tuple(sorted(c.items())) for c in cands}
+    assert len(keys) == len(cands)
+    assert all(c["es_pair_set_max"] == -1 for c in cands if c["es_lane_set_max"] != 128)
+    assert {c["es_pair_set_max"] for c in cands if c["es_lane_set_max"] == 128} == \
+        set(autotune.ES_SPACE["es_pair_set_max"])
+    try:
+        for c in cands:
+            cg.set_tuning(**c)
+    finally:
+        cg.set_tuning()

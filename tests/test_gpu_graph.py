@@ -144,3 +144,35 @@ def test_sweep_graph_stage_times_agree_with_events(gpu):
         assert tg[k] > 0 and te[k] > 0, k
         assert abs(tg[k] - te[k]) <= 0.25 * te[k] + 0.05, (k, tg[k], te[k])
     assert tg["ms_total"] >= tg["ms_metric"] + tg["ms_es"]
+
+
+def test_autotuner_short_search(gpu, tmp_path):
+    # a short search on a small ragged workload: the knobs it returns are
+    # from the search space, the tuned run is never slower than the defaults
+    # as the tuner measured them, the cache entry is written, and the tuned
+    # shapes give the default shapes' statistics bit for bit
+    import torch
+    from cudagsea_amd import autotune
+    w = workloads.make("c3", num_genes=4000, num_samples=100, num_sets=600, num_perms=512)
+    w["metrics"] = ["naive_diff_of_classes", "stable_t_test"]
+    path = str(tmp_path / "cache.json")
+    key, res = autotune.tune_workload(w, sample_perms=256, path=path)
+    assert res["tuned_ms"] <= res["default_ms"]
+    k = res["knobs"]
+    if k:
+        assert k["es_lane_set_max"] in autotune.ES_SPACE["es_lane_set_max"]
+        assert k["es_pair_set_max"] in autotune.ES_SPACE["es_pair_set_max"]
+        assert k["es_waves"] in autotune.ES_SPACE["es_waves"]
+    assert autotune.load_cache(path)[key]["knobs"] == k
+    X = dev(w["X"])
+    st = []
+    for knobs in ({}, k):
+        gpu.set_tuning(**knobs)
+        try:
+            s = torch.empty((2, 5, w["G"]), dtype=torch.float32, device="cuda")
+            gpu.sweep_device(X, w["labels"], dev(w["set_offsets"]), dev(w["set_genes"]),
+                             512, w["metrics"], s)
+            st.append(s.cpu().numpy())
+        finally:
+            gpu.set_tuning()
+    assert bits_equal(st[0], st[1]).all()
