@@ -166,3 +166,55 @@ def test_cli_results_match_library_and_oracle(gpu, tmp_path):
                         "stable_signal2noise")
     np.testing.assert_allclose(got[:, 0], st[0], atol=2e-6)
     np.testing.assert_allclose(got[:, 1], st[1], rtol=1e-5, atol=2e-6)
+
+
+@pytest.mark.gpu
+def test_cli_ascending_double_dump(gpu, tmp_path):
+    """-order ascending, -precision double, -dump and -gpu 0: RESULT lines
+    equal the library's fp64 ascending run on the same parsed data, the dump
+    file has the reference's name and holds that run's ES matrix, and the
+    observed column matches the oracle."""
+    import oracle
+    import parity
+    import cudagsea_amd as cg
+    need_cli()
+    genes = hallmark_symbols()[:1500]
+    S = 200
+    gct = tmp_path / "e.gct"
+    write_gct(gct, genes, S, seed=5)
+    P = 128
+    prefix = tmp_path / "run"
+    print("cli ascending double dump", len(genes), S, P, flush=True)
+    rc, out = run("-res", gct, "-cls", CLS, "-gmx", GMT, "-nperm", P,
+                  "-metric", "twopass_t_test", "-order", "ascending",
+                  "-precision", "double", "-dump", prefix, "-gpu", 0)
+    assert rc == 0, out[-2000:]
+    rows = [RESULT.match(l) for l in out.splitlines() if l.startswith("RESULT:")]
+    assert len(rows) == 50 and all(rows)
+    got = np.array([[float(m.group(i)) for i in range(1, 5)] for m in rows])
+    names = [m.group(5) for m in rows]
+    X = np.loadtxt(gct, skiprows=3, usecols=range(2, 2 + S), dtype=np.float64)
+    labels = np.array(open(CLS).read().split("\n")[2].split(), dtype=np.uint8)
+    sets = {}
+    for line in open(GMT):
+        t = line.rstrip("\n").split("\t")
+        sets[t[0]] = [x.strip() for x in t[2:] if x.strip()]
+    sym = {g: i for i, g in enumerate(genes)}
+    off, gl = [0], []
+    for n in names:
+        gl.extend(sorted({sym[s] for s in sets[n] if s in sym}))
+        off.append(len(gl))
+    off, gl = np.array(off, np.int32), np.array(gl, np.int32)
+    Xs = np.ascontiguousarray(X.T)            # sample-major
+    st, es = cg.run(Xs, labels, off, gl, P, "twopass_t_test", ascending=True,
+                    double=True, want_es=True)
+    np.testing.assert_allclose(got, st[:4].T, atol=1.5e-6)   # printed with 6 decimals
+    dump = "%s_%d_%d_64.es" % (prefix, len(names), P)
+    assert os.path.exists(dump), os.listdir(tmp_path)
+    d = cg.read_dump(dump)
+    assert d.dtype == np.float64 and d.shape == (len(names), P)
+    assert parity.bits_equal(d, es).all()
+    r = oracle.correlate(Xs, labels, P, "twopass_t_test", sample_major=True, double=True)
+    sk, idx = oracle.pathway_sort(r, ascending=True)
+    want, hi, lo = oracle.scores_ext(sk, idx, off, gl)
+    parity.assert_es(d, want, hi, lo)
