@@ -2,7 +2,8 @@
 
 The RNG is pinned against the reference's own generator: rngpu/rngpu.hpp
 compiled as shipped by g++ (oracle/ref), whose outputs are committed in
-tests/golden/ref_vectors.npz, plus the survey's known answer.  The rest of the
+tests/golden/ref_vectors.npz, plus the known answer that permutation 1 of
+[0..9] is [5 1 2 3 6 4 8 9 0 7].  The rest of the
 reference's path cannot be built here (its headers need nvcc's keywords, the
 CUDA runtime or CUB), so the other oracle functions are pinned by closed forms
 on exactly representable data, float64 brute force written from the
