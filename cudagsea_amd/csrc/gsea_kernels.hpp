@@ -63,10 +63,40 @@ __global__ void k_large_genes(const int32_t *__restrict__ off,
                               const int32_t *__restrict__ genes,
                               const int4 *__restrict__ desc,
                               uint16_t *__restrict__ out) {
+    // k_es gathers pos[id + 1] for 32 consecutive entries per half-wavefront
+    // access; pos is 16-bit, so entry id + 1 = g1 sits in LDS bank
+    // (g1 >> 1) mod 32 up to a rotation that is the same for every entry.
+    // The entries are stored sorted by (r, bank), r being an entry's index
+    // among its set's entries of the same bank in set order: the r-th round
+    // holds one entry of each bank with more than r entries, so an aligned
+    // window of 32 entries spans at most the end of one round and the start
+    // of the next, and its banks are distinct but where the two overlap.
+    // The order of a set's entries changes nothing but the order of its N_R
+    // summands.  An entry's position is the number of (r, bank) keys below
+    // its own: sum over banks c of min(cnt_c, r) + #{c < bank : cnt_c > r}.
+    __shared__ int cnt[32];
     const int4 d = desc[blockIdx.x];
     const int32_t *src = genes + off[d.x];
+    if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+    __syncthreads();
     for (int t = threadIdx.x; t < d.z; t += blockDim.x)
-        out[d.y + t] = (uint16_t)(src[t] + 1);
+        atomicAdd(&cnt[((src[t] + 1) >> 1) & 31], 1);
+    __syncthreads();
+    if (threadIdx.x >= 32) return;
+    // thread b places the entries of bank b, in set order
+    const int b = threadIdx.x;
+    int r = 0;
+    for (int t = 0; t < d.z; ++t) {
+        const int g1 = src[t] + 1;
+        if (((g1 >> 1) & 31) != b) continue;
+        int p = 0;
+        for (int c = 0; c < 32; ++c) {
+            const int n = cnt[c];
+            p += min(n, r) + (c < b && n > r ? 1 : 0);
+        }
+        out[d.y + p] = (uint16_t)g1;
+        ++r;
+    }
 }
 
 // gene ids of the small sets, laid out [t][slot] so a wavefront's 64 lanes
