@@ -304,6 +304,38 @@ def test_rank_es_bitmap_capacity_boundaries(gpu, N):
     parity.assert_es(got, want, hi, lo)
 
 
+@pytest.mark.parametrize("dbl", [False, True])
+def test_rank_es_large_set_bank_layouts(gpu, dbl):
+    # large sets' gene ids are stored round by round, one id per LDS bank of
+    # the 16-bit pos row ((id + 1) >> 1 mod 32): ids that all share one bank
+    # (rounds of one id), two banks, every bank once (one full round), ragged
+    # bank counts, and contiguous blocks; each set's ES against the oracle,
+    # and a second identical call bit-equal to the first
+    print("large-set bank layouts", dbl, flush=True)
+    N = 20000
+    rng = np.random.default_rng(11)
+    ids = np.arange(N, dtype=np.int32)
+    bank = ((ids + 1) >> 1) & 31
+    sets = [ids[bank == 7][:600],
+            ids[(bank == 0) | (bank == 31)][:900],
+            np.sort(np.concatenate([rng.choice(ids[bank == b], 9, replace=False)
+                                    for b in range(32)])),
+            np.sort(np.concatenate([rng.choice(ids[bank == b], 1 + 3 * b, replace=False)
+                                    for b in range(32)])),
+            np.arange(5000, 7000, dtype=np.int32),
+            np.arange(N - 257, N, dtype=np.int32),
+            np.sort(rng.choice(N, 1999, replace=False)).astype(np.int32)]
+    sets = [np.asarray(s, np.int32) for s in sets]
+    w = small("c1", num_genes=N, num_perms=8, num_sets=2)
+    w["set_offsets"], w["set_genes"] = csr(sets)
+    w["G"] = len(sets)
+    metric = "stable_signal2noise"
+    got, want, hi, lo = _es_stage_case(gpu, w, 8, metric, dbl)
+    parity.assert_es(got, want, hi, lo)
+    again, _, _, _ = _es_stage_case(gpu, w, 8, metric, dbl)
+    assert bits_equal(got, again).all()
+
+
 # ----------------------------------------------------------- whole path ----
 @pytest.mark.parametrize("dbl", [False, True])
 def test_full_path_c1(gpu, dbl):
