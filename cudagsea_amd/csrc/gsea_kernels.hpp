@@ -74,25 +74,32 @@ __global__ void k_large_genes(const int32_t *__restrict__ off,
     // The order of a set's entries changes nothing but the order of its N_R
     // summands.  An entry's position is the number of (r, bank) keys below
     // its own: sum over banks c of min(cnt_c, r) + #{c < bank : cnt_c > r}.
+    // 256 threads: thread (b, s) = (tid % 32, tid / 32) handles the ids of
+    // bank b in segment s, the s-th eighth of the set; a prefix over the
+    // segments gives each thread the round index of its first id
+    constexpr int kSeg = 8;
+    __shared__ int cs[kSeg][32];
     __shared__ int cnt[32];
     const int4 d = desc[blockIdx.x];
     const int32_t *src = genes + off[d.x];
-    if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+    const int b = threadIdx.x & 31, s = threadIdx.x >> 5;
+    const int len = (d.z + kSeg - 1) / kSeg;
+    const int t0 = min(s * len, d.z), t1 = min(t0 + len, d.z);
+    int n = 0;
+    for (int t = t0; t < t1; ++t) n += (((src[t] + 1) >> 1) & 31) == b ? 1 : 0;
+    cs[s][b] = n;
     __syncthreads();
-    for (int t = threadIdx.x; t < d.z; t += blockDim.x)
-        atomicAdd(&cnt[((src[t] + 1) >> 1) & 31], 1);
-    __syncthreads();
-    if (threadIdx.x >= 32) return;
-    // thread b places the entries of bank b, in set order
-    const int b = threadIdx.x;
     int r = 0;
-    for (int t = 0; t < d.z; ++t) {
+    for (int s2 = 0; s2 < s; ++s2) r += cs[s2][b];
+    if (s == kSeg - 1) cnt[b] = r + n;
+    __syncthreads();
+    for (int t = t0; t < t1; ++t) {
         const int g1 = src[t] + 1;
         if (((g1 >> 1) & 31) != b) continue;
         int p = 0;
         for (int c = 0; c < 32; ++c) {
-            const int n = cnt[c];
-            p += min(n, r) + (c < b && n > r ? 1 : 0);
+            const int m = cnt[c];
+            p += min(m, r) + (c < b && m > r ? 1 : 0);
         }
         out[d.y + p] = (uint16_t)g1;
         ++r;
