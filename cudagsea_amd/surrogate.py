@@ -9,7 +9,7 @@ divisor of the permutation count) at which that precision meets the target.
 The engine's budgeted run then scores every set for its budget, checks the
 measured statistics and scores for all permutations every set whose FDR lies
 near the threshold or whose measured NES precision misses the target.  The
-plain full run stays the reference the allocation is judged against.
+plain full run stays the reference the allocation is checked against.
 
 Everything numerical runs in the HIP library: features (gsea_set_features),
 training targets (gsea_null_moments), the network's forward/backward passes and
