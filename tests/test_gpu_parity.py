@@ -83,6 +83,37 @@ def test_metric_tiles_per_kind_do_not_change_results(gpu):
         gpu.set_tuning()
 
 
+def test_metric_welford_fp32_wide_inputs(gpu):
+    # the Welford metrics divide by the running count at every sample: many
+    # samples (counts up to 3,000), values over 2^-20..2^40 with zeros,
+    # cancelling pairs, near-constant and constant genes, and single values at
+    # 2^-30, 2^50 and inf; bit-exact with the oracle
+    import torch
+    rng = np.random.default_rng(21)
+    N, S, P = 640, 3000, 16
+    sgn = rng.choice([-1.0, 1.0], (S, N))
+    X = sgn * np.exp2(rng.uniform(-20, 40, (S, N)))
+    X[:, 0:64] = 1e4 + rng.standard_normal((S, 64)) * 1e-2       # near-constant
+    X[:, 64:128] = np.where(rng.random((S, 64)) < 0.7, 0.0, X[:, 64:128])
+    X[::2, 128:192] = 3.0
+    X[1::2, 128:192] = -3.0                                      # cancelling
+    X[:, 192:224] = 2.0 ** -20
+    X[:, 224:256] = -(2.0 ** 40)
+    X[:, 256:272] = 7.25                                         # constant
+    X[5, 300] = 2.0 ** -30                                       # out of range
+    X[7, 340] = 2.0 ** 50
+    X[9, 380] = np.inf
+    X = X.astype(np.float32)
+    lab = (rng.random(S) < 0.4).astype(np.uint8)
+    Xd = dev(X)
+    for m in ("stable_signal2noise", "stable_t_test"):
+        want = oracle.correlate(X, lab, P, m, shift=3, sample_major=True)
+        r = torch.empty((P, N), dtype=torch.float32, device="cuda")
+        gpu.metric_device(Xd, lab, P, m, r, shift=3)
+        eq = bits_equal(r.cpu().numpy(), want)
+        assert eq.all(), (m, int((~eq).sum()), np.argwhere(~eq)[:3])
+
+
 def test_metric_log_bitwise_over_wide_range(gpu):
     # drive the log2-ratio metric through ratios spanning many binades,
     # subnormal and special results included
