@@ -721,6 +721,13 @@ __device__ __forceinline__ void es_large_set(const RankArgs<T> &a, const float *
                                              const uint16_t *pos, AF &A,
                                              uint32_t *bm, int64_t gp, int s,
                                              int o0, int k, double decay, int lane) {
+    if (k == 0) {
+        // no hits (an empty set reaches this path when every set is scored
+        // by a wavefront): the walk only falls, to N*decay at its end, as on
+        // the one-lane path
+        if (lane == 0) a.es[(size_t)gp * a.G + s] = (T)((double)a.N * decay);
+        return;
+    }
     const uint16_t *sg = a.large_genes + o0;
     double nr = 0.0;
     es_mark_set<T>(sg, pos, A, bm, k, a.N, lane, nr);

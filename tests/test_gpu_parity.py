@@ -335,6 +335,36 @@ def test_rank_es_bitmap_capacity_boundaries(gpu, N):
     parity.assert_es(got, want, hi, lo)
 
 
+@pytest.mark.parametrize("knobs", [
+    dict(es_lane_set_max=-1),                               # every set by a wavefront
+    dict(es_lane_set_max=8, es_pair_set_max=-1, es_waves=12),
+    dict(es_lane_set_max=32, es_waves=5),
+    dict(es_lane_set_max=128, es_pair_set_max=256, es_waves=1),
+])
+def test_rank_es_work_splits(gpu, knobs):
+    # the ES stage's work split is a launch-shape choice: sets of every size
+    # class, the empty set and single genes, through each path the knobs
+    # select (tiny sets on the wavefront path, no lane pairs, few wavefronts
+    # per workgroup), each per element against the oracle
+    print("es work split", knobs, flush=True)
+    rng = np.random.default_rng(17)
+    N = 5000
+    w = small("c1", num_genes=N, num_perms=12, num_sets=2)
+    sizes = [0, 1, 2, 7, 8, 9, 31, 33, 64, 65, 127, 128, 129, 200, 256, 257, 600, 2100]
+    sets = [np.sort(rng.choice(N, k, replace=False)).astype(np.int32) for k in sizes]
+    sets += [np.array([0], np.int32), np.array([N - 1], np.int32),
+             np.arange(N - 300, N, dtype=np.int32)]
+    w["set_offsets"], w["set_genes"] = csr(sets)
+    w["G"] = len(sets)
+    try:
+        gpu.set_tuning(**knobs)
+        for asc in (False, True):
+            got, want, hi, lo = _es_stage_case(gpu, w, 12, "onepass_signal2noise", asc=asc)
+            parity.assert_es(got, want, hi, lo)
+    finally:
+        gpu.set_tuning()
+
+
 @pytest.mark.parametrize("dbl", [False, True])
 def test_rank_es_large_set_bank_layouts(gpu, dbl):
     # large sets' gene ids are stored round by round, one id per LDS bank of
