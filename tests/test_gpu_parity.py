@@ -58,6 +58,25 @@ def test_metric_bit_exact_all_metrics(gpu, config, N, S, dbl):
         assert eq.all(), (m, int((~eq).sum()), np.argwhere(~eq)[:3])
 
 
+@pytest.mark.parametrize("dbl", [False, True])
+def test_metric_bit_exact_many_samples(gpu, dbl):
+    # 5,000 samples: no [S][16] tile fits the 64 KiB the kernel stages, so X
+    # is read from global memory (one gene per lane); all 14 metrics
+    import torch
+    print("metric parity, 5000 samples", "fp64" if dbl else "fp32", flush=True)
+    w = small("c2", num_genes=300, num_samples=5000, num_sets=4, num_perms=8)
+    dt = np.float64 if dbl else np.float32
+    X = w["X"].astype(dt)
+    Xd = dev(X)
+    for m in oracle.METRICS:
+        want = oracle.correlate(X, w["labels"], 8, m, shift=2, sample_major=True, double=dbl)
+        r = torch.empty((8, 300), dtype=torch.float64 if dbl else torch.float32,
+                        device="cuda")
+        gpu.metric_device(Xd, w["labels"], 8, m, r, shift=2)
+        eq = bits_equal(r.cpu().numpy(), want)
+        assert eq.all(), (m, int((~eq).sum()), np.argwhere(~eq)[:3])
+
+
 def test_metric_tiles_per_kind_do_not_change_results(gpu):
     # every launch shape computes the same arithmetic: per-kind tiles and
     # permutation blocks give the default shapes' bits
@@ -341,12 +360,13 @@ def test_rank_es_bitmap_capacity_boundaries(gpu, N):
     dict(es_lane_set_max=32, es_waves=5),
     dict(es_lane_set_max=128, es_pair_set_max=256, es_waves=1),
 ])
-def test_rank_es_work_splits(gpu, knobs):
+@pytest.mark.parametrize("dbl", [False, True])
+def test_rank_es_work_splits(gpu, knobs, dbl):
     # the ES stage's work split is a launch-shape choice: sets of every size
     # class, the empty set and single genes, through each path the knobs
     # select (tiny sets on the wavefront path, no lane pairs, few wavefronts
     # per workgroup), each per element against the oracle
-    print("es work split", knobs, flush=True)
+    print("es work split", knobs, "fp64" if dbl else "fp32", flush=True)
     rng = np.random.default_rng(17)
     N = 5000
     w = small("c1", num_genes=N, num_perms=12, num_sets=2)
@@ -359,7 +379,8 @@ def test_rank_es_work_splits(gpu, knobs):
     try:
         gpu.set_tuning(**knobs)
         for asc in (False, True):
-            got, want, hi, lo = _es_stage_case(gpu, w, 12, "onepass_signal2noise", asc=asc)
+            got, want, hi, lo = _es_stage_case(gpu, w, 12, "onepass_signal2noise", dbl=dbl,
+                                               asc=asc)
             parity.assert_es(got, want, hi, lo)
     finally:
         gpu.set_tuning()
