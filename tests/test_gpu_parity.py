@@ -333,6 +333,67 @@ def test_rank_es_pair_sets_tie_rule(gpu, dbl):
         assert bits_equal(got, want).all(), (top, got.ravel(), want.ravel())
 
 
+@pytest.mark.parametrize("dbl", [False, True])
+@pytest.mark.parametrize("k", [64, 200, 512])
+def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
+    # |largest| == |smallest| running sum on the one-lane path (64 genes), the
+    # pair path (200) and the wavefront path (512): set genes have |metric| 2
+    # (each hit adds 1/k) among N = k + 16k genes (each miss subtracts
+    # 1/(16k)).  Equal metrics rank by gene index.
+    # (max first) the top k/2 and bottom k/2 ranks are hits: +1/2 after hit
+    # k/2-1, -1/2 just before hit k/2, so the ES is +1/2;
+    # (min first) 8k misses (-1/2), then the k hits (+1/2), then 8k misses:
+    # -1/2 comes first, so the ES is -1/2.
+    # With k a power of two every step is dyadic and every sum exact: the GPU
+    # must tie as the reference does and keep the first, bit for bit.  With
+    # k = 200 the steps are not dyadic; the GPU walk carries c = |r|/N_R
+    # unrounded in double, so its two extremes can differ in the last bits
+    # where the reference's tie exactly, and either may win: the ES bar with
+    # its sign flip at a tie applies (parity.es_check).
+    dyadic = (k & (k - 1)) == 0
+
+    def check(got, want, hi, lo, what):
+        if dyadic:
+            assert bits_equal(got, want).all(), (what, got.ravel(), want.ravel())
+        else:
+            parity.assert_es(got, want, hi, lo, what)
+
+    N, P = 17 * k, 4
+    dt = np.float64 if dbl else np.float32
+    h = k // 2
+    a = np.full(N, -2.0)
+    a[:h] = 2.0
+    a[h:N - h] = np.linspace(1.9, -1.9, N - 2 * h)
+    set_a = np.r_[0:h, N - h:N]
+    b = np.empty(N)
+    b[:8 * k] = np.linspace(10.0, 3.0, 8 * k)
+    b[8 * k:9 * k] = 2.0
+    b[9 * k:] = np.linspace(1.9, -1.9, N - 9 * k)
+    set_b = np.arange(8 * k, 9 * k)
+    for base, genes, es in ((a, set_a, 0.5), (b, set_b, -0.5)):
+        assert len(genes) == k
+        r = np.tile(base.astype(dt), (P, 1))
+        got, want, hi, lo = _es_rows_case(gpu, r, [genes.astype(np.int32)])
+        assert (hi == 0.5).all() and (lo == -0.5).all(), "the case must tie exactly"
+        assert (want == es).all(), (es, want.ravel())
+        check(got, want, hi, lo, (k, es))
+    # ties two ranks apart: all N = 2k metrics are 2, so ranks follow gene
+    # index and a hit and a miss move the walk by the same step x = 1/k.
+    # "hit, miss, miss, (hit, miss)*, hit" peaks at +x on rank 0 and first
+    # falls to -x on rank 2 (ES +x); its mirror "miss, hit, hit, (miss,
+    # hit)*, miss" gives -x first (ES -x)
+    N = 2 * k
+    r = np.full((P, N), 2.0, dtype=dt)
+    hm = np.r_[[1, 0, 0], np.tile([1, 0], k - 2), [1]].astype(bool)
+    for hits, sign in ((hm, 1.0), (~hm, -1.0)):
+        genes = np.flatnonzero(hits).astype(np.int32)
+        assert len(genes) == k
+        got, want, hi, lo = _es_rows_case(gpu, r, [genes])
+        assert (hi == -lo).all() and (hi > 0).all(), "the case must tie exactly"
+        assert (np.sign(want) == sign).all(), (sign, want.ravel())
+        check(got, want, hi, lo, (k, sign))
+
+
 @pytest.mark.parametrize("N", [2047, 2048, 4095, 4096, 32767, 32768])
 def test_rank_es_bitmap_capacity_boundaries(gpu, N):
     # the ES kernel's per-wavefront bitmap holds ranks 0..N (rank N is the
