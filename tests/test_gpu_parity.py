@@ -357,6 +357,10 @@ def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
             assert bits_equal(got, want).all(), (what, got.ravel(), want.ravel())
         else:
             parity.assert_es(got, want, hi, lo, what)
+        # ascending order of -r ranks the genes as descending order of r
+        # (equal values by gene index either way): the same scores, bit for bit
+        mirror = _es_rows_case(gpu, -r, sets, asc=True)[0]
+        assert bits_equal(mirror, got).all(), (what, mirror.ravel(), got.ravel())
 
     N, P = 17 * k, 4
     dt = np.float64 if dbl else np.float32
@@ -373,7 +377,8 @@ def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
     for base, genes, es in ((a, set_a, 0.5), (b, set_b, -0.5)):
         assert len(genes) == k
         r = np.tile(base.astype(dt), (P, 1))
-        got, want, hi, lo = _es_rows_case(gpu, r, [genes.astype(np.int32)])
+        sets = [genes.astype(np.int32)]
+        got, want, hi, lo = _es_rows_case(gpu, r, sets)
         assert (hi == 0.5).all() and (lo == -0.5).all(), "the case must tie exactly"
         assert (want == es).all(), (es, want.ravel())
         check(got, want, hi, lo, (k, es))
@@ -388,7 +393,8 @@ def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
     for hits, sign in ((hm, 1.0), (~hm, -1.0)):
         genes = np.flatnonzero(hits).astype(np.int32)
         assert len(genes) == k
-        got, want, hi, lo = _es_rows_case(gpu, r, [genes])
+        sets = [genes]
+        got, want, hi, lo = _es_rows_case(gpu, r, sets)
         assert (hi == -lo).all() and (hi > 0).all(), "the case must tie exactly"
         assert (np.sign(want) == sign).all(), (sign, want.ravel())
         check(got, want, hi, lo, (k, sign))
