@@ -1,6 +1,10 @@
 """Per-element parity bars between the HIP path and the CPU oracle.
 
-ES and NES: within 1e-5 relative (|d| <= 1e-5 |x| + 1e-6), fp32 and fp64.
+ES: within half the tie window (below) absolute, 2^-21 in fp32 and 2^-33 in
+fp64: the GPU walk's drift from the reference's (bounded in tie_window) plus
+a one-ulp difference of N_R and the Kahan error of the reference's sums in T
+(an ulp or two) stay below it.  NES and the ES-derived statistics: within
+1e-5 relative (|d| <= 1e-5 |x| + 1e-6), fp32 and fp64.
 The one allowed exception is a sign flip where the oracle's walk has two
 candidates of nearly equal magnitude: its largest positive running sum `hi`
 and its most negative one `lo` (oracle.scores_ext).  The reference keeps the
@@ -69,9 +73,9 @@ def es_check(got, want, hi, lo):
     l = np.asarray(lo, dtype=np.float64)
     want = np.asarray(want, dtype=np.float64)
     alt = np.where(want == h, l, h)
-    direct = close(got, want)
+    direct = close(got, want, rtol=0.0, atol=win / 2)
     near = (np.abs(np.abs(h) - np.abs(l)) <= win) & (np.maximum(np.abs(h), np.abs(l)) > 0)
-    flip = ~direct & near & close(got, alt)
+    flip = ~direct & near & close(got, alt, rtol=0.0, atol=win / 2)
     return direct | flip, flip
 
 
