@@ -11,12 +11,12 @@ and its most negative one `lo` (oracle.scores_ext).  The reference keeps the
 one of larger magnitude (the first of two equal ones); when ||hi| - |lo|| is
 within tie_window (2^-20 in fp32, 2^-32 in fp64, derived there) the two
 implementations' roundings may pick either, so the GPU ES must then be
-within the bar of the other candidate.  The count bounds below keep the
-wider test (gap within the ES bar), which only widens them.
+within the bar of the other candidate.
 
 NP and FWER are counts over permutations: exact, except for comparisons of two
-values that lie within the ES bar of each other (the tie counts below), plus,
-for FWER, the null scores that may flip as above.
+values that lie within one tie window (NP, on ES) or within the NES bar
+(FWER, on normalised scores) of each other, plus, for FWER, the null scores
+that may flip as above.
 
 FDR: the counts it is made of (null scores beyond NES, null scores on NES's
 side of 0, observed NES beyond NES, observed NES on its side of 0) are
@@ -43,12 +43,6 @@ def close(a, b, rtol=ES_RTOL, atol=ES_ATOL):
     return ok | (np.isnan(a) & np.isnan(b))
 
 
-def flippable(hi, lo):
-    """Walks whose two candidates have magnitudes within the ES bar."""
-    hi = np.asarray(hi, dtype=np.float64)
-    lo = np.asarray(lo, dtype=np.float64)
-    m = np.maximum(np.abs(hi), np.abs(lo))
-    return (np.abs(np.abs(hi) - np.abs(lo)) <= ES_RTOL * m + ES_ATOL) & (m > 0)
 
 
 def tie_window(dtype):
@@ -63,6 +57,16 @@ def tie_window(dtype):
     return 2.0 ** -20 if np.dtype(dtype) == np.float32 else 2.0 ** -32
 
 
+def flippable(hi, lo):
+    """Walks whose two candidates have magnitudes within tie_window: the
+    scores whose sign may differ from the oracle's."""
+    win = tie_window(np.asarray(hi).dtype)
+    hi = np.asarray(hi, dtype=np.float64)
+    lo = np.asarray(lo, dtype=np.float64)
+    m = np.maximum(np.abs(hi), np.abs(lo))
+    return (np.abs(np.abs(hi) - np.abs(lo)) <= win) & (m > 0)
+
+
 def es_check(got, want, hi, lo):
     """(ok, flipped): ok per element; flipped marks the elements that passed
     as a justified sign flip: the oracle's two candidates lie within
@@ -74,8 +78,7 @@ def es_check(got, want, hi, lo):
     want = np.asarray(want, dtype=np.float64)
     alt = np.where(want == h, l, h)
     direct = close(got, want, rtol=0.0, atol=win / 2)
-    near = (np.abs(np.abs(h) - np.abs(l)) <= win) & (np.maximum(np.abs(h), np.abs(l)) > 0)
-    flip = ~direct & near & close(got, alt, rtol=0.0, atol=win / 2)
+    flip = ~direct & flippable(hi, lo) & close(got, alt, rtol=0.0, atol=win / 2)
     return direct | flip, flip
 
 
@@ -133,9 +136,11 @@ def stats_bounds(es, hi, lo):
     z_alt = alt * fac[:, None]
     nes = z[:, 0]
 
-    # NP: |ES_p| >= |ES_0| (p = 0 included), comparisons within the bar
+    # NP: |ES_p| >= |ES_0| (p = 0 included); each GPU ES is within half the
+    # tie window of the oracle's, so comparisons within one window may go
+    # either way
     a = np.abs(e)
-    b = _band(a[:, :1])
+    b = tie_window(np.asarray(hi).dtype)
     np_lo = (a > a[:, :1] + b).sum(axis=1) + 1
     np_hi = (a >= a[:, :1] - b).sum(axis=1)
 
