@@ -359,7 +359,7 @@ def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
             parity.assert_es(got, want, hi, lo, what)
         # ascending order of -r ranks the genes as descending order of r
         # (equal values by gene index either way): the same scores, bit for bit
-        mirror = _es_rows_case(gpu, -r, sets, asc=True)[0]
+        mirror = _es_rows_case(gpu, -r, sets, asc=True)[0][:1]
         assert bits_equal(mirror, got).all(), (what, mirror.ravel(), got.ravel())
 
     N, P = 17 * k, 4
@@ -377,8 +377,16 @@ def test_rank_es_tie_rule_lane_and_wavefront(gpu, k, dbl):
     for base, genes, es in ((a, set_a, 0.5), (b, set_b, -0.5)):
         assert len(genes) == k
         r = np.tile(base.astype(dt), (P, 1))
-        sets = [genes.astype(np.int32)]
+        # 60 more sets in the same call (5..700 genes): they are scored by
+        # the wavefronts and lanes that scored the tie, after it, so a tie
+        # branch that left state behind (bitmap bits) shows in their scores
+        rng = np.random.default_rng(k)
+        extra = [np.sort(rng.choice(N, int(m), replace=False)).astype(np.int32)
+                 for m in rng.integers(5, min(700, N), 60)]
+        sets = [genes.astype(np.int32)] + extra
         got, want, hi, lo = _es_rows_case(gpu, r, sets)
+        parity.assert_es(got[1:], want[1:], hi[1:], lo[1:], ("sets after the tie", k))
+        got, want, hi, lo = got[:1], want[:1], hi[:1], lo[:1]
         assert (hi == 0.5).all() and (lo == -0.5).all(), "the case must tie exactly"
         assert (want == es).all(), (es, want.ravel())
         check(got, want, hi, lo, (k, es))
